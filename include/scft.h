@@ -1,0 +1,166 @@
+/*
+ * scft.h -- C ABI of the MI355X SCFT / complex-Langevin hot path.
+ *
+ * The reference (rar-ensemble/aligned-nanoplate-separation-and-rotation) is one
+ * MPI + FFTW C++ program whose hot path communicates through process globals
+ * (code/globals.h:37-167).  This header replaces those seams with a context
+ * object and plain C types: interleaved complex<double> arrays (re, im) of
+ * length M in the reference layout, linear index x + Nx*(y + Ny*z), x fastest
+ * (code/array_utils.cpp:80-87).  Every call returns SCFT_OK (0) or a negative
+ * error code; nothing calls exit().
+ *
+ * Threading: one host thread per context.  Calls on one context are ordered
+ * on the context's HIP stream; the scft_get_* calls synchronise.
+ */
+#ifndef SCFT_H
+#define SCFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SCFT_OK            0
+#define SCFT_EINVAL       -1   /* bad argument or parameter set            */
+#define SCFT_ENOMEM       -2   /* device or host allocation failed         */
+#define SCFT_EDEVICE      -3   /* HIP runtime error                        */
+#define SCFT_ENAN         -4   /* H or Q is NaN (code/calc_h.cpp:38-61,
+                                  code/simulate.cpp:100-111)               */
+#define SCFT_EUNSUPPORTED -5   /* grid size / option not built            */
+#define SCFT_EIO          -6   /* file could not be read or written       */
+
+/* ---- precision of the propagator / FFT stage --------------------------- */
+#define SCFT_PREC_FP64 0       /* plain complex128 everywhere              */
+#define SCFT_PREC_FP32 1       /* complex64 propagator and FFTs; fields,
+                                  Q, log Q, H and residual stay FP64       */
+
+/* ---- plate shapes ------------------------------------------------------ */
+#define SCFT_NP_SPHERE 1       /* reference np_type 1 (not on this path)   */
+#define SCFT_NP_ROD    2       /* reference np_type 2: 1/4 erfc*erfc,
+                                  code/initialize.cpp:578-612              */
+#define SCFT_NP_SQUARE 3       /* extension: square plate, 1/8 erfc^3      */
+
+/*
+ * Simulation parameters.  Field order follows the positional bcp.input file
+ * read by read_input() (code/read_input.cpp:31-236).  chi values are stored
+ * signed, exactly as written in the file; the negative-chi flags of the
+ * reference (code/read_input.cpp:53-85) are derived from the sign.
+ */
+typedef struct scft_params {
+  int dim;                         /* 2 or 3 (reference: compile-time Dim)  */
+  /* main parameters, code/read_input.cpp:31-47 */
+  int Nda, Ndb;
+  double phiH;
+  int Nah;
+  double a_smear;
+  double rho0;
+  /* interaction parameters, code/read_input.cpp:53-87 */
+  double chi_ab, chi_bc, chi_ac;
+  double kappa;
+  /* initial-condition flags, code/read_input.cpp:103-108 */
+  int ic_flag[2];
+  double ic_pre[2], ic_dir[2], ic_period[2];
+  /* numerical parameters, code/read_input.cpp:114-135 */
+  int Nx[3];
+  double L[3];
+  double lam_imag, lam_real;
+  int itermax, print_freq, sample_freq, sample_wait;
+  double error_tol;
+  int update_scheme;               /* 0 = Euler, 1 = 1S                    */
+  int do_CL;
+  /* film, code/read_input.cpp:141-146 */
+  int do_film;
+  double wallT, wallXi, top_wall_lamA, top_wall_lamB, bot_wall_lamA,
+      bot_wall_lamB;
+  /* channel, code/read_input.cpp:155-158 */
+  int do_channel, hor_dir, vert_dir;
+  double wall_width, channel_width, channel_xi;
+  /* nanoparticles, code/read_input.cpp:182-236 */
+  int n_exp_nr, do_fld_np, np_type, np_chem;
+  double sigma;
+  int Ng, Nu;
+  double np_frac;
+  double L_nr, R_nr, xi_nr;
+  double exp_nr_c[2][3];           /* fractional centres (0..1)            */
+  double exp_nr_u[2][3];           /* unit orientation, normalised on read */
+  /* extension: seed of the counter-based noise generator (CL noise) and of
+   * ic_flag == -1 random starts.  The reference seeds ran2 from the clock
+   * (code/main.cpp:20). */
+  uint64_t seed;
+} scft_params;
+
+/* Scalars fixed at set-up (code/initialize.cpp:45-81, 249-355). */
+typedef struct scft_sysinfo {
+  int M;                           /* grid points                          */
+  int N;                           /* Nda + Ndb                            */
+  double V, Vf, dx[3];
+  double V_1_exp_np, np_frac, phiD, phiG, nP, ng_per_np, nD;
+  double Rg2, fD;
+} scft_sysinfo;
+
+typedef struct scft_ctx scft_ctx;
+
+/* Parse a bcp.input file (code/read_input.cpp:7-242) for a run in `dim`
+ * dimensions: each grid/box/centre/orientation line holds `dim` values and
+ * any further number on the line is skipped, as fgets() does there. */
+int scft_read_input(const char* path, int dim, scft_params* out);
+
+/* Create a context holding `n_systems` systems on HIP device `device`.
+ * `params` points to n_systems structs (boxes may differ).  Performs the
+ * set-up of initialize_1/initialize_2 (code/initialize.cpp:45-412): smearing
+ * and bond kernels, plate densities, volume bookkeeping, Debye functions and
+ * the initial fields of init_fields (code/initialize.cpp:419-565). */
+int scft_create(const scft_params* params, int n_systems, int device,
+                int precision, scft_ctx** out);
+int scft_destroy(scft_ctx* ctx);
+int scft_get_info(scft_ctx* ctx, int sys, scft_sysinfo* out);
+
+/* Field exchange with host memory: w+, wAB+, wAB- (code/globals.h:76-90).
+ * Replaces the .res override of read_resume_files (code/io_utils.cpp:183-240). */
+int scft_set_fields(scft_ctx* ctx, int sys, const double* wpl,
+                    const double* wabp, const double* wabm);
+int scft_get_fields(scft_ctx* ctx, int sys, double* wpl, double* wabp,
+                    double* wabm);
+/* Diblock densities rhoda, rhodb in real space (code/globals.h:99-100). */
+int scft_get_density(scft_ctx* ctx, int sys, double* rhoA, double* rhoB);
+/* Smeared fields smwa, smwb (code/calc_poly_density.cpp:93-104). */
+int scft_get_smeared(scft_ctx* ctx, int sys, double* smwa, double* smwb);
+/* Plate density rho_exp_nr (code/initialize.cpp:210-239). */
+int scft_get_cavity(scft_ctx* ctx, int sys, double* rho);
+/* Single-chain partition function Qd of the last density evaluation. */
+int scft_get_Q(scft_ctx* ctx, int sys, double* Q_re_im);
+
+/* FFT seam, fft_fwd_wrapper / fft_bck_wrapper (code/fft_wrappers.cpp:4-67):
+ * dir = -1 forward, scaled by 1/M; dir = +1 backward, unscaled.  `inout` is a
+ * host array of M interleaved complex values of system `sys`'s grid. */
+int scft_fft(scft_ctx* ctx, int sys, double* inout, int dir);
+
+/* Stage calls on every system of the context. */
+/* calc_poly_density() (code/calc_poly_density.cpp:37-122, 167-191): field
+ * assembly, smearing, diblock propagators (code/diblock_discrete.cpp:4-95),
+ * Qd and the q.q-dagger density sum. */
+int scft_calc_poly_density(scft_ctx* ctx);
+/* update_1s() (code/1s_update.cpp:7-352), ending in calc_poly_density(). */
+int scft_update_1s(scft_ctx* ctx);
+/* update_Euler() (code/Euler_update.cpp:5-213), ending in calc_poly_density(). */
+int scft_update_euler(scft_ctx* ctx);
+/* calc_H() (code/calc_h.cpp:9-64); H_re_im receives 2*n_systems doubles.
+ * Returns SCFT_ENAN if any H is NaN. */
+int scft_calc_H(scft_ctx* ctx, double* H_re_im);
+
+/* The loop body of simulate() (code/simulate.cpp:85-157) for n_iter
+ * iterations of the update selected by update_scheme, without H or output.
+ * Captured into a HIP graph after the first call. */
+int scft_iterate(scft_ctx* ctx, int n_iter);
+
+/* Text of the last error on this thread. */
+const char* scft_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCFT_H */
