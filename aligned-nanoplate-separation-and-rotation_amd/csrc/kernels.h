@@ -1,0 +1,21 @@
+// Launch wrappers of scft_kernels.hip (internal to libscft_amd.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace scft {
+
+struct PassDesc;
+struct SysDev;
+
+// prec: 0 = complex128, 1 = complex64.  mode: MODE_CONV / MODE_FWD / MODE_INV.
+// plane: k_plane (x and y of one xy plane per block) or k_line (one axis).
+hipError_t launch_pass(int prec, int mode, bool plane, const PassDesc* d, int njobs, int gridx,
+                       int tb, size_t lds, int step, hipStream_t st);
+hipError_t launch_assemble(const SysDev* s, int nsys, int maxM, hipStream_t st);
+hipError_t launch_exp_prop(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);
+hipError_t launch_density(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);
+hipError_t launch_update(int scheme, const SysDev* s, int nsys, int maxM, hipStream_t st);
+hipError_t launch_reduce(int what, int prec, const SysDev* s, int nsys, int nblk, hipStream_t st);
+
+}  // namespace scft

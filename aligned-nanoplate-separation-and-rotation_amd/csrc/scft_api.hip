@@ -1,0 +1,1094 @@
+// Host side of libscft_amd.so: context, set-up and the C ABI of include/scft.h.
+//
+// One context holds n systems (possibly of different box and grid) on one
+// GPU and one HIP stream.  Everything the hot path touches lives in HBM:
+//   per system, complex128: w+, wAB+, wAB-, rhoA, rhoB, wa/smwa, wb/smwb,
+//     rho_np, exp_nrH, an FFT scratch; real128: h(k), X(k), X2(k);
+//   per system, propagator precision: q and q-dagger as N*M contiguous
+//     arrays (segment-major), exp(-wA), exp(-wB).
+// FFT-style work is described once, at create, as "programs": lists of
+// launches over device tables of PassDesc.  The contour step is a kernel
+// argument, so a whole iteration is a fixed launch sequence that is
+// captured once into a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/scft.h"
+#include "kernels.h"
+#include "scft_dev.hpp"
+
+using scft::PassDesc;
+using scft::Plan;
+using scft::SysDev;
+typedef std::complex<double> cd;
+
+namespace {
+
+const double kPi = 3.14159265358979323846264338327950288;
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCK(x)                                                                   \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      return fail(SCFT_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// radix plans and twiddles
+// ---------------------------------------------------------------------------
+bool make_plan(int n, Plan* p) {
+  p->n = n;
+  p->npass = 0;
+  p->radices = 0;
+  int m = n;
+  const int order[] = {8, 4, 2, 3, 5, 7};
+  for (int r : order) {
+    while (m % r == 0) {
+      if (p->npass == 8) return false;
+      p->radices |= (unsigned)r << (4 * p->npass);
+      p->npass++;
+      m /= r;
+    }
+  }
+  return m == 1;
+}
+
+// largest register footprint of a pass: ceil((n B / R) / T) * R values
+int pass_values(const Plan& p, int nB, int T) {
+  int worst = 0;
+  for (int i = 0; i < p.npass; i++) {
+    const int R = p.radix(i);
+    const int total = nB / R;
+    const int u = (total + T - 1) / T;
+    worst = std::max(worst, u * R);
+  }
+  return worst;
+}
+
+constexpr int kMaxV = 32;
+constexpr size_t kMaxLds = 160 * 1024;
+
+int pick_tb(const Plan& a, const Plan* b, int nB) {
+  for (int T : {256, 512, 1024}) {
+    if (pass_values(a, nB, T) <= kMaxV && (!b || pass_values(*b, nB, T) <= kMaxV)) return T;
+  }
+  return 0;
+}
+
+// k of index n on an axis of length Nx and box L, code/array_utils.cpp:248-251
+inline double kval(int n, int Nx, double L) {
+  return (double(n) < double(Nx) / 2.0) ? 2 * kPi * double(n) / L : 2 * kPi * double(n - Nx) / L;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct Launch {
+  bool plane;
+  int mode, prec;
+  PassDesc* d;
+  int njobs, gridx, tb;
+  size_t lds;
+};
+
+struct Program {
+  std::vector<Launch> launches;
+};
+
+struct SysHost {
+  scft_params p;
+  scft_sysinfo info;
+  int nx, ny, nz, M, N, Na;
+  double L[3], dx[3], dV;
+  int neg_ab;
+  double chi_ab;
+  SysDev dev;  // host copy of the device table row
+  // 1D tables, per axis: bond (prop precision) and smear (FP64), device
+  void* bond_g[3];
+  void* smear_g[3];
+  std::vector<void*> allocs;
+};
+
+struct scft_ctx {
+  int device = 0;
+  int prec = 0;
+  size_t esz = 16;  // bytes per propagator complex
+  hipStream_t st = nullptr;
+  std::vector<SysHost> sys;
+  SysDev* d_sys = nullptr;
+  int maxM = 0, maxN = 0, nblk = 1;
+  std::map<int, size_t> tw_off;  // n -> element offset in the twiddle tables
+  void* tw64 = nullptr;
+  void* tw32 = nullptr;
+  std::vector<void*> allocs;
+  Program prog_smear, prog_prop, prog_fwd, prog_inv, prog_noise;
+  std::vector<Program> prog_fft_fwd, prog_fft_inv;  // per system, on its scratch
+  std::vector<void*> scratch;                       // per system FP64 scratch
+  hipGraphExec_t graph = nullptr;
+  int graph_scheme = -1;
+};
+
+namespace {
+
+int dalloc(scft_ctx* c, std::vector<void*>& list, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(SCFT_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  list.push_back(*p);
+  (void)c;
+  return SCFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// program building
+// ---------------------------------------------------------------------------
+struct JobSpec {
+  int sys;
+  int prec;
+  void* in;
+  void* out;
+  long in_step, out_step;
+  int in_soff, out_soff;
+  const void* post_a;
+  const void* post_b;
+  int post_switch;
+  int steps_active;
+  int gkind;  // 0 none, 1 bond, 2 smear
+  double scale;
+};
+
+enum StageKind { ST_PLANE = 0, ST_X = 1, ST_Y = 2, ST_Z = 3 };
+
+const void* tw_ptr(scft_ctx* c, int n, int prec) {
+  const size_t off = c->tw_off.at(n);
+  return prec == 0 ? (const void*)((double2*)c->tw64 + off) : (const void*)((float2*)c->tw32 + off);
+}
+
+int build_program(scft_ctx* c, int mode, const std::vector<JobSpec>& jobs, Program* out) {
+  // stage lists per job
+  struct Item {
+    int job;
+    int stage;  // position in the job's sequence
+    int kind;
+    bool last;
+  };
+  std::vector<std::vector<int>> seq(jobs.size());
+  for (size_t j = 0; j < jobs.size(); j++) {
+    const SysHost& s = c->sys[jobs[j].sys];
+    const size_t esz = jobs[j].prec == 0 ? 16 : 8;
+    Plan px, py;
+    make_plan(s.nx, &px);
+    make_plan(s.ny, &py);
+    const bool fits = (size_t)s.nx * s.ny * esz <= kMaxLds &&
+                      pick_tb(px, &py, s.nx * s.ny) != 0 && s.nx * s.ny > 1;
+    std::vector<int>& q = seq[j];
+    if (fits) q.push_back(ST_PLANE);
+    else {
+      if (s.nx > 1) q.push_back(ST_X);
+      if (s.ny > 1) q.push_back(ST_Y);
+    }
+    if (s.nz > 1) q.push_back(ST_Z);
+  }
+  // group by (stage index, kind, prec)
+  std::map<std::tuple<int, int, int>, std::vector<Item>> groups;
+  for (size_t j = 0; j < jobs.size(); j++)
+    for (size_t k = 0; k < seq[j].size(); k++)
+      groups[{(int)k, seq[j][k], jobs[j].prec}].push_back(
+          {(int)j, (int)k, seq[j][k], k + 1 == seq[j].size()});
+  std::vector<PassDesc> all;
+  std::vector<Launch> launches;
+  std::vector<size_t> starts;
+  for (auto& kv : groups) {
+    const int kind = std::get<1>(kv.first);
+    const int prec = std::get<2>(kv.first);
+    const size_t esz = prec == 0 ? 16 : 8;
+    Launch L{};
+    L.plane = kind == ST_PLANE;
+    L.mode = mode;
+    L.prec = prec;
+    L.njobs = (int)kv.second.size();
+    starts.push_back(all.size());
+    for (const Item& it : kv.second) {
+      const JobSpec& J = jobs[it.job];
+      const SysHost& s = c->sys[J.sys];
+      PassDesc d{};
+      if (it.stage == 0) {
+        d.in = J.in;
+        d.in_step = J.in_step;
+        d.in_soff = J.in_soff;
+      } else {
+        d.in = J.out;
+        d.in_step = J.out_step;
+        d.in_soff = J.out_soff;
+      }
+      d.out = J.out;
+      d.out_step = J.out_step;
+      d.out_soff = J.out_soff;
+      d.post_a = J.post_a;
+      d.post_b = J.post_b;
+      d.post_switch = J.post_switch;
+      d.steps_active = J.steps_active;
+      d.nx = s.nx;
+      d.ny = s.ny;
+      d.nz = s.nz;
+      d.last = it.last ? 1 : 0;
+      d.scale = J.scale;
+      auto gsel = [&](int axis) -> const void* {
+        if (mode != scft::MODE_CONV) return nullptr;
+        return J.gkind == 1 ? s.bond_g[axis] : s.smear_g[axis];
+      };
+      if (kind == ST_PLANE) {
+        make_plan(s.nx, &d.p0);
+        make_plan(s.ny, &d.p1);
+        d.tw0 = tw_ptr(c, s.nx, prec);
+        d.tw1 = tw_ptr(c, s.ny, prec);
+        d.g0 = gsel(0);
+        d.g1 = gsel(1);
+        d.ntiles = s.nz;
+        const int tb = pick_tb(d.p0, &d.p1, s.nx * s.ny);
+        L.tb = std::max(L.tb, tb);
+        L.lds = std::max(L.lds, (size_t)s.nx * s.ny * esz);
+        L.gridx = std::max(L.gridx, s.nz);
+      } else {
+        const int axis = kind - ST_X;
+        const int n = axis == 0 ? s.nx : (axis == 1 ? s.ny : s.nz);
+        d.axis = axis;
+        make_plan(n, &d.p0);
+        d.tw0 = tw_ptr(c, n, prec);
+        d.g0 = gsel(axis);
+        const int nlines = s.M / n;
+        const int budget = prec == 0 ? 2048 : 4096;
+        int tl = std::max(1, budget / n);
+        if (tl >= 8) tl &= ~7;
+        tl = std::min(tl, 64);
+        tl = std::min(tl, nlines);
+        int tb = 0;
+        while (tl >= 1) {
+          tb = pick_tb(d.p0, nullptr, tl * n);
+          if (tb && (size_t)tl * n * esz <= kMaxLds) break;
+          tl = tl > 8 ? tl - 8 : tl - 1;
+        }
+        if (tl < 1 || !tb) return fail(SCFT_EUNSUPPORTED, "no tile shape for line length " + std::to_string(n));
+        d.tl = tl;
+        d.ntiles = (nlines + tl - 1) / tl;
+        L.tb = std::max(L.tb, tb);
+        L.lds = std::max(L.lds, (size_t)tl * n * esz);
+        L.gridx = std::max(L.gridx, d.ntiles);
+      }
+      all.push_back(d);
+    }
+    launches.push_back(L);
+  }
+  // upload the descriptor table
+  PassDesc* dd = nullptr;
+  if (!all.empty()) {
+    void* p;
+    int rc = dalloc(c, c->allocs, &p, all.size() * sizeof(PassDesc));
+    if (rc) return rc;
+    dd = (PassDesc*)p;
+    HIPCK(hipMemcpy(dd, all.data(), all.size() * sizeof(PassDesc), hipMemcpyHostToDevice));
+  }
+  // order launches by stage index (map order is stage-major already)
+  for (size_t i = 0; i < launches.size(); i++) launches[i].d = dd + starts[i];
+  out->launches = launches;
+  return SCFT_OK;
+}
+
+int run_program(scft_ctx* c, const Program& p, int step) {
+  for (const Launch& L : p.launches) {
+    HIPCK(scft::launch_pass(L.prec, L.mode, L.plane, L.d, L.njobs, L.gridx, L.tb, L.lds, step, c->st));
+  }
+  return SCFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// set-up of one system: initialize_1 / initialize_2 (code/initialize.cpp)
+// ---------------------------------------------------------------------------
+void plate_axes(const double* u, double* e1, double* e2) {
+  double c[3] = {0.0, u[2], -u[1]};  // u x X
+  double n = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  if (n < 1e-12) {
+    c[0] = -u[2];  // u x Y
+    c[1] = 0.0;
+    c[2] = u[0];
+    n = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  }
+  for (int d = 0; d < 3; d++) e2[d] = c[d] / n;
+  e1[0] = e2[1] * u[2] - e2[2] * u[1];
+  e1[1] = e2[2] * u[0] - e2[0] * u[2];
+  e1[2] = e2[0] * u[1] - e2[1] * u[0];
+}
+
+// Plate density at grid point (n0, n1, n2): explicit_nanorod
+// (code/initialize.cpp:578-612) for np_type 2, the square plate otherwise.
+double plate_rho(const SysHost& s, int np, const int* nn) {
+  const scft_params& p = s.p;
+  const int dim = p.dim;
+  double x[3], c[3], dr[3] = {0, 0, 0};
+  for (int d = 0; d < dim; d++) {
+    x[d] = double(nn[d]) * s.dx[d];
+    c[d] = p.exp_nr_c[np][d] * s.L[d];
+    dr[d] = x[d] - c[d];
+    if (dr[d] >= 0.5 * s.L[d]) dr[d] -= s.L[d];
+    else if (dr[d] < -0.5 * s.L[d]) dr[d] += s.L[d];
+  }
+  const double* u = p.exp_nr_u[np];
+  double ud = 0;
+  for (int d = 0; d < dim; d++) ud += u[d] * dr[d];
+  const double a = std::erfc((std::fabs(ud) - 0.5 * p.L_nr) / p.xi_nr);
+  if (p.np_type == SCFT_NP_SQUARE && dim == 3) {
+    double e1[3], e2[3];
+    plate_axes(u, e1, e2);
+    const double a1 = e1[0] * dr[0] + e1[1] * dr[1] + e1[2] * dr[2];
+    const double a2 = e2[0] * dr[0] + e2[1] * dr[1] + e2[2] * dr[2];
+    return 0.125 * a * std::erfc((std::fabs(a1) - p.R_nr) / p.xi_nr) *
+           std::erfc((std::fabs(a2) - p.R_nr) / p.xi_nr);
+  }
+  double uc;
+  if (dim == 2) uc = std::fabs(u[0] * dr[1] - u[1] * dr[0]);
+  else {
+    const double c0 = u[1] * dr[2] - u[2] * dr[1], c1 = u[2] * dr[0] - u[0] * dr[2],
+                 c2 = u[0] * dr[1] - u[1] * dr[0];
+    uc = std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  }
+  return 0.25 * a * std::erfc((std::fabs(uc) - p.R_nr) / p.xi_nr);
+}
+
+// Numerical Recipes ran2 + polar Box-Muller, for ic_flag == -1 starts
+// (code/ran2.cpp:32-90 uses the same published generator).
+struct Ran2 {
+  long idum, idum2 = 123456789, iy = 0, iv[32];
+  explicit Ran2(long seed) : idum(seed) {}
+  double next() {
+    const long m1 = 2147483563, m2 = 2147483399, a1 = 40014, a2 = 40692, q1 = 53668,
+               q2 = 52774, r1 = 12211, r2 = 3791, mm1 = m1 - 1, ndiv = 1 + mm1 / 32;
+    if (idum <= 0) {
+      idum = (-idum < 1) ? 1 : -idum;
+      idum2 = idum;
+      for (int j = 39; j >= 0; j--) {
+        long h = idum / q1;
+        idum = a1 * (idum - h * q1) - h * r1;
+        if (idum < 0) idum += m1;
+        if (j < 32) iv[j] = idum;
+      }
+      iy = iv[0];
+    }
+    long h = idum / q1;
+    idum = a1 * (idum - h * q1) - h * r1;
+    if (idum < 0) idum += m1;
+    h = idum2 / q2;
+    idum2 = a2 * (idum2 - h * q2) - h * r2;
+    if (idum2 < 0) idum2 += m2;
+    const int j = (int)(iy / ndiv);
+    iy = iv[j] - idum2;
+    iv[j] = idum;
+    if (iy < 1) iy += mm1;
+    const double t = iy * (1.0 / m1);
+    return t > 1.0 - 1.2e-7 ? 1.0 - 1.2e-7 : t;
+  }
+};
+
+int setup_system(scft_ctx* c, int si) {
+  SysHost& s = c->sys[si];
+  const scft_params& p = s.p;
+  if (p.dim != 2 && p.dim != 3) return fail(SCFT_EINVAL, "dim must be 2 or 3");
+  if (p.Nda < 0 || p.Ndb < 0 || p.Nda + p.Ndb < 2) return fail(SCFT_EINVAL, "Nda + Ndb must be >= 2");
+  if (p.do_film || p.do_channel || p.do_fld_np || p.phiH > 0 || p.np_type == SCFT_NP_SPHERE)
+    return fail(SCFT_EUNSUPPORTED, "films, channels, field nanoparticles, homopolymer and spheres are not on this path");
+  if (p.sigma > 0.0 && p.Ng > 0)
+    return fail(SCFT_EUNSUPPORTED, "grafted chains are not on this path yet");
+  if (p.chi_bc != 0.0 || p.chi_ac != 0.0)
+    return fail(SCFT_EUNSUPPORTED, "BC/AC field pairs carry graft densities only and are not on this path yet");
+  s.nx = p.Nx[0];
+  s.ny = p.Nx[1];
+  s.nz = p.dim == 3 ? p.Nx[2] : 1;
+  if (s.nx < 1 || s.ny < 1 || s.nz < 1) return fail(SCFT_EINVAL, "bad grid");
+  s.M = s.nx * s.ny * s.nz;
+  s.N = p.Nda + p.Ndb;
+  s.Na = p.Nda;
+  double V = 1.0;
+  s.dV = 1.0;
+  for (int d = 0; d < 3; d++) {
+    const int n = d == 0 ? s.nx : (d == 1 ? s.ny : s.nz);
+    s.L[d] = d < p.dim ? p.L[d] : 1.0;
+    s.dx[d] = s.L[d] / double(n);
+    if (d < p.dim) {
+      V *= s.L[d];
+      s.dV *= s.dx[d];
+    }
+  }
+  for (int n : {s.nx, s.ny, s.nz}) {
+    Plan pl;
+    if (!make_plan(n, &pl)) return fail(SCFT_EUNSUPPORTED, "grid size " + std::to_string(n) + " has a prime factor above 7");
+  }
+  s.neg_ab = p.chi_ab < 0;
+  s.chi_ab = std::fabs(p.chi_ab);
+  const int M = s.M, N = s.N;
+  const double fD = double(p.Nda) / double(N);
+  const double Rg2 = double(N - 1) / 6.0;
+  const double a2 = p.a_smear * p.a_smear;
+
+  // plate density and volume bookkeeping, code/initialize.cpp:210-355
+  std::vector<cd> rho_np(M, cd(0.0, 0.0));
+  for (int np = 0; np < p.n_exp_nr && np < 2; np++) {
+    for (int i = 0; i < M; i++) {
+      int nn[3] = {i % s.nx, (i / s.nx) % s.ny, i / (s.nx * s.ny)};
+      rho_np[i] += plate_rho(s, np, nn);
+    }
+  }
+  cd sum = 0.0;
+  for (int i = 0; i < M; i++) sum += rho_np[i];
+  for (int d = 0; d < p.dim; d++) sum *= s.dx[d];
+  const double V_exp = sum.real();
+  const double Vf = V;
+  const double np_frac = V_exp / Vf;
+  const double V1 = p.n_exp_nr > 0 ? V_exp / double(p.n_exp_nr) : 0.0;
+  const double nP = p.n_exp_nr > 0 ? V_exp / V1 : 0.0;
+  const double phiD = 1.0 - p.phiH - np_frac;
+  const double nD = phiD * p.rho0 * Vf / N;
+  scft_sysinfo& inf = s.info;
+  memset(&inf, 0, sizeof inf);
+  inf.M = M;
+  inf.N = N;
+  inf.V = V;
+  inf.Vf = Vf;
+  for (int d = 0; d < 3; d++) inf.dx[d] = s.dx[d];
+  inf.V_1_exp_np = V1;
+  inf.np_frac = np_frac;
+  inf.phiD = phiD;
+  inf.phiG = 0.0;
+  inf.nP = nP;
+  inf.ng_per_np = 0.0;
+  inf.nD = nD;
+  inf.Rg2 = Rg2;
+  inf.fD = fD;
+
+  // k-space coefficients: h(k) (code/initialize.cpp:97-102), Debye sums
+  // (code/calc_debye.cpp:5-51) folded into X(k) = phiD rho0 N h^2 (gaa + 2 gab + gbb)
+  std::vector<double> hk(M), X(M), X2(M);
+  for (int i = 0; i < M; i++) {
+    const int n0 = i % s.nx, n1 = (i / s.nx) % s.ny, n2 = i / (s.nx * s.ny);
+    double k2 = 0.0;
+    const int nn[3] = {n0, n1, n2};
+    const int Ns[3] = {s.nx, s.ny, s.nz};
+    for (int d = 0; d < p.dim; d++) {
+      const double k = kval(nn[d], Ns[d], s.L[d]);
+      k2 += k * k;
+    }
+    const double h = std::exp(-k2 * a2 / 2.0);
+    const double x = k2 * Rg2;
+    double gaa, gbb, gab;
+    if (i == 0) {
+      gaa = fD * fD;
+      gbb = (1.0 - fD) * (1.0 - fD);
+      gab = fD * (1.0 - fD);
+    } else {
+      gaa = 2.0 * (std::exp(-x * fD) + fD * x - 1.0) / x / x;
+      gbb = 2.0 * (std::exp(-x * (1.0 - fD)) + (1.0 - fD) * x - 1.0) / x / x;
+      gab = (1.0 - std::exp(-x * fD)) * (1.0 - std::exp(-x * (1.0 - fD))) / x / x;
+    }
+    hk[i] = h;
+    X[i] = phiD * p.rho0 * N * h * h * (gaa + 2.0 * gab + gbb);
+    X2[i] = phiD * p.rho0 * N * h * h * (gaa - gbb);
+  }
+
+  // initial fields, init_fields (code/initialize.cpp:419-565)
+  std::vector<cd> wpl(M, 0.0), wabp(M, 0.0), wabm(M, 0.0);
+  {
+    const cd I(0.0, 1.0);
+    Ran2 rng(-(long)((p.seed % 2147483000ULL) + 1));
+    for (int i = 0; i < M; i++) {
+      const int nn[3] = {i % s.nx, (i / s.nx) % s.ny, i / (s.nx * s.ny)};
+      double x[3];
+      for (int d = 0; d < 3; d++) {
+        x[d] = s.dx[d] * nn[d];
+        if (x[d] > s.L[d] / 2.0) x[d] -= s.L[d];
+        else if (x[d] <= -s.L[d] / 2.0) x[d] += s.L[d];
+      }
+      if (p.ic_flag[0] < -1) {
+        if (rho_np[i].real() > 0.5) {
+          wabm[i] = 0.0;
+        } else {
+          cd v = 1.0;
+          const cd ph = s.chi_ab != 0 ? (s.neg_ab ? I : cd(1.0)) : cd(1.0);
+          for (int j = 0; j < 2; j++) {
+            const int dir = std::max(0, std::min(2, int(p.ic_dir[j])));
+            const double arg = 2.0 * kPi * p.ic_period[j] * x[dir] / s.L[dir];
+            if (p.ic_flag[j] == -2) v *= p.ic_pre[j] * std::tanh(std::cos(arg) / 0.2) * ph;
+            else if (p.ic_flag[j] == -3) v *= p.ic_pre[j] * std::tanh(std::sin(arg) / 0.2) * ph;
+          }
+          wabm[i] = v;
+        }
+      } else if (p.ic_flag[0] == -1) {
+        if (s.chi_ab != 0) {
+          wabp[i] = p.ic_pre[0] * (rng.next() - 0.5) * (s.neg_ab ? cd(1.0) : I);
+          wabm[i] = p.ic_pre[0] * (rng.next() - 0.5) * (s.neg_ab ? I : cd(1.0));
+        }
+        wpl[i] = p.ic_pre[0] * (rng.next() - 0.5) * I;
+      }
+    }
+  }
+
+  // device allocations
+  const size_t m16 = (size_t)M * 16, m8 = (size_t)M * 8;
+  const size_t pe = c->esz;
+  SysDev& d = s.dev;
+  memset(&d, 0, sizeof d);
+  void* p_;
+  int rc;
+#define ALLOC(field, bytes)                                 \
+  rc = dalloc(c, s.allocs, &p_, (bytes));                   \
+  if (rc) return rc;                                        \
+  d.field = (decltype(d.field))p_;
+  ALLOC(wpl, m16);
+  ALLOC(wabp, m16);
+  ALLOC(wabm, m16);
+  ALLOC(rhoA, m16);
+  ALLOC(rhoB, m16);
+  ALLOC(wa, m16);
+  ALLOC(wb, m16);
+  ALLOC(rho_np, m16);
+  ALLOC(exp_nrH, m16);
+  ALLOC(hhat_k, m8);
+  ALLOC(Xk, m8);
+  ALLOC(X2k, m8);
+  ALLOC(q, (size_t)N * M * pe);
+  if (p.Nda != 0 && p.Nda != N) {
+    ALLOC(qd, (size_t)N * M * pe);
+  } else {
+    d.qd = d.q;
+  }
+  ALLOC(expA, (size_t)M * pe);
+  ALLOC(expB, (size_t)M * pe);
+  if (p.do_CL) {
+    ALLOC(etap, 2 * m16);
+    ALLOC(etam, m16);
+  }
+  ALLOC(red, scft::RED_NSLOTS * sizeof(double2));
+  d.nblk = std::min(1024, (M + 255) / 256);
+  ALLOC(partials, (size_t)scft::RED_NSLOTS * d.nblk * sizeof(double2));
+  rc = dalloc(c, s.allocs, &p_, m16);
+  if (rc) return rc;
+  c->scratch[si] = p_;
+#undef ALLOC
+  HIPCK(hipMemset(d.red, 0, scft::RED_NSLOTS * sizeof(double2)));
+  d.M = M;
+  d.N = N;
+  d.Na = p.Nda;
+  d.nx = s.nx;
+  d.ny = s.ny;
+  d.nz = s.nz;
+  d.dim = p.dim;
+  d.neg_ab = s.neg_ab;
+  d.do_cl = p.do_CL;
+  d.np_chem = p.np_chem;
+  d.nD = nD;
+  d.V = V;
+  d.dV = s.dV;
+  d.rho0 = p.rho0;
+  d.kappa = p.kappa;
+  d.chi_ab = s.chi_ab;
+  d.lam_imag = p.lam_imag;
+  d.lam_real = p.lam_real;
+  d.fA = fD;
+  HIPCK(hipMemcpy(d.rho_np, rho_np.data(), m16, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(d.hhat_k, hk.data(), m8, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(d.Xk, X.data(), m8, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(d.X2k, X2.data(), m8, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(d.wpl, wpl.data(), m16, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(d.wabp, wabp.data(), m16, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(d.wabm, wabm.data(), m16, hipMemcpyHostToDevice));
+  HIPCK(hipMemset(d.rhoA, 0, m16));
+  HIPCK(hipMemset(d.rhoB, 0, m16));
+
+  // 1D tables along each axis: bond exp(-k^2/6)/n, smear exp(-k^2 a^2/2)/n
+  for (int ax = 0; ax < 3; ax++) {
+    const int n = ax == 0 ? s.nx : (ax == 1 ? s.ny : s.nz);
+    std::vector<double> b(n), h(n);
+    std::vector<float> bf(n);
+    for (int m = 0; m < n; m++) {
+      const double k = ax < p.dim ? kval(m, n, s.L[ax]) : 0.0;
+      b[m] = std::exp(-k * k / 6.0) / n;
+      h[m] = std::exp(-k * k * a2 / 2.0) / n;
+      bf[m] = (float)b[m];
+    }
+    rc = dalloc(c, s.allocs, &p_, n * pe / 2);
+    if (rc) return rc;
+    s.bond_g[ax] = p_;
+    if (c->prec == 0) HIPCK(hipMemcpy(p_, b.data(), n * 8, hipMemcpyHostToDevice));
+    else HIPCK(hipMemcpy(p_, bf.data(), n * 4, hipMemcpyHostToDevice));
+    rc = dalloc(c, s.allocs, &p_, n * 8);
+    if (rc) return rc;
+    s.smear_g[ax] = p_;
+    HIPCK(hipMemcpy(p_, h.data(), n * 8, hipMemcpyHostToDevice));
+  }
+  return SCFT_OK;
+}
+
+int build_twiddles(scft_ctx* c) {
+  std::vector<int> ns;
+  for (auto& s : c->sys)
+    for (int n : {s.nx, s.ny, s.nz})
+      if (std::find(ns.begin(), ns.end(), n) == ns.end()) ns.push_back(n);
+  size_t tot = 0;
+  for (int n : ns) {
+    c->tw_off[n] = tot;
+    tot += n;
+  }
+  std::vector<double2> t64(tot);
+  std::vector<float2> t32(tot);
+  for (int n : ns) {
+    const size_t o = c->tw_off[n];
+    for (int m = 0; m < n; m++) {
+      // exp(-2 pi i m / n) from long double sin/cos
+      const long double a = -2.0L * 3.14159265358979323846264338327950288L * m / n;
+      t64[o + m] = make_double2((double)cosl(a), (double)sinl(a));
+      t32[o + m] = make_float2((float)cosl(a), (float)sinl(a));
+    }
+  }
+  int rc = dalloc(c, c->allocs, &c->tw64, tot * sizeof(double2));
+  if (rc) return rc;
+  rc = dalloc(c, c->allocs, &c->tw32, tot * sizeof(float2));
+  if (rc) return rc;
+  HIPCK(hipMemcpy(c->tw64, t64.data(), tot * sizeof(double2), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(c->tw32, t32.data(), tot * sizeof(float2), hipMemcpyHostToDevice));
+  return SCFT_OK;
+}
+
+JobSpec job(int sys, int prec, void* in, void* out, int gkind, double scale) {
+  JobSpec j{};
+  j.sys = sys;
+  j.prec = prec;
+  j.in = in;
+  j.out = out;
+  j.gkind = gkind;
+  j.scale = scale;
+  return j;
+}
+
+int build_programs(scft_ctx* c) {
+  const int ns = (int)c->sys.size();
+  std::vector<JobSpec> smear, prop, fwd, inv;
+  for (int i = 0; i < ns; i++) {
+    SysHost& s = c->sys[i];
+    SysDev& d = s.dev;
+    smear.push_back(job(i, 0, d.wa, d.wa, 2, 1.0));
+    smear.push_back(job(i, 0, d.wb, d.wb, 2, 1.0));
+    // propagator step: q[step-1] -> q[step], exp(-w) on the last store
+    JobSpec q = job(i, c->prec, d.q, d.q, 1, 1.0);
+    q.in_step = q.out_step = s.M;
+    q.in_soff = -1;
+    q.out_soff = 0;
+    q.post_a = d.expA;
+    q.post_b = d.expB;
+    q.post_switch = s.Na > 0 ? s.Na : 0;
+    q.steps_active = s.N;
+    prop.push_back(q);
+    if (d.qd != d.q) {
+      JobSpec qd = q;
+      qd.in = qd.out = d.qd;
+      qd.post_a = d.expB;
+      qd.post_b = d.expA;
+      qd.post_switch = s.N - s.Na;
+      prop.push_back(qd);
+    }
+    const double inv_m = 1.0 / double(s.M);
+    for (void* a : {(void*)d.rhoA, (void*)d.rhoB, (void*)d.wpl, (void*)d.wabm, (void*)d.wabp})
+      fwd.push_back(job(i, 0, a, a, 0, inv_m));
+    for (void* a : {(void*)d.wpl, (void*)d.wabm, (void*)d.wabp}) inv.push_back(job(i, 0, a, a, 0, 1.0));
+  }
+  int rc;
+  if ((rc = build_program(c, scft::MODE_CONV, smear, &c->prog_smear))) return rc;
+  if ((rc = build_program(c, scft::MODE_CONV, prop, &c->prog_prop))) return rc;
+  if ((rc = build_program(c, scft::MODE_FWD, fwd, &c->prog_fwd))) return rc;
+  if ((rc = build_program(c, scft::MODE_INV, inv, &c->prog_inv))) return rc;
+  c->prog_fft_fwd.resize(ns);
+  c->prog_fft_inv.resize(ns);
+  for (int i = 0; i < ns; i++) {
+    std::vector<JobSpec> f = {job(i, 0, c->scratch[i], c->scratch[i], 0, 1.0 / double(c->sys[i].M))};
+    std::vector<JobSpec> b = {job(i, 0, c->scratch[i], c->scratch[i], 0, 1.0)};
+    if ((rc = build_program(c, scft::MODE_FWD, f, &c->prog_fft_fwd[i]))) return rc;
+    if ((rc = build_program(c, scft::MODE_INV, b, &c->prog_fft_inv[i]))) return rc;
+  }
+  return SCFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stages
+// ---------------------------------------------------------------------------
+int stage_poly_density(scft_ctx* c) {
+  const int ns = (int)c->sys.size();
+  HIPCK(scft::launch_assemble(c->d_sys, ns, c->maxM, c->st));
+  int rc = run_program(c, c->prog_smear, 0);
+  if (rc) return rc;
+  if (c->prec == 1) HIPCK(scft::launch_reduce(scft::RED_SHIFT, 0, c->d_sys, ns, c->nblk, c->st));
+  HIPCK(scft::launch_exp_prop(c->prec, c->d_sys, ns, c->maxM, c->st));
+  for (int step = 1; step < c->maxN; step++) {
+    rc = run_program(c, c->prog_prop, step);
+    if (rc) return rc;
+  }
+  HIPCK(scft::launch_reduce(scft::RED_Q, c->prec, c->d_sys, ns, c->nblk, c->st));
+  HIPCK(scft::launch_density(c->prec, c->d_sys, ns, c->maxM, c->st));
+  return SCFT_OK;
+}
+
+int stage_update(scft_ctx* c, int scheme) {
+  int rc = run_program(c, c->prog_fwd, 0);
+  if (rc) return rc;
+  HIPCK(scft::launch_update(scheme, c->d_sys, (int)c->sys.size(), c->maxM, c->st));
+  rc = run_program(c, c->prog_inv, 0);
+  if (rc) return rc;
+  return stage_poly_density(c);
+}
+
+int check_ctx(scft_ctx* c, int sys) {
+  if (!c) return fail(SCFT_EINVAL, "null context");
+  if (sys < 0 || sys >= (int)c->sys.size()) return fail(SCFT_EINVAL, "system index out of range");
+  HIPCK(hipSetDevice(c->device));
+  return SCFT_OK;
+}
+
+int copy_out(scft_ctx* c, void* host, const void* dev, size_t bytes) {
+  HIPCK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->st));
+  HIPCK(hipStreamSynchronize(c->st));
+  return SCFT_OK;
+}
+
+int copy_in(scft_ctx* c, void* dev, const void* host, size_t bytes) {
+  HIPCK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->st));
+  HIPCK(hipStreamSynchronize(c->st));
+  return SCFT_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* scft_last_error(void) { return g_err.c_str(); }
+
+int scft_create(const scft_params* params, int n_systems, int device, int precision, scft_ctx** out) {
+  if (!params || !out || n_systems < 1) return fail(SCFT_EINVAL, "bad arguments to scft_create");
+  if (precision != SCFT_PREC_FP64 && precision != SCFT_PREC_FP32) return fail(SCFT_EINVAL, "bad precision");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(SCFT_EDEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(SCFT_EINVAL, "device index out of range");
+  HIPCK(hipSetDevice(device));
+  scft_ctx* c = new scft_ctx();
+  c->device = device;
+  c->prec = precision;
+  c->esz = precision == SCFT_PREC_FP64 ? 16 : 8;
+  c->sys.resize(n_systems);
+  c->scratch.resize(n_systems);
+  int rc = SCFT_OK;
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(SCFT_EDEVICE, "hipStreamCreate failed");
+  }
+  for (int i = 0; i < n_systems && rc == SCFT_OK; i++) {
+    c->sys[i].p = params[i];
+    rc = setup_system(c, i);
+  }
+  if (rc == SCFT_OK) rc = build_twiddles(c);
+  if (rc == SCFT_OK) {
+    for (auto& s : c->sys) {
+      c->maxM = std::max(c->maxM, s.M);
+      c->maxN = std::max(c->maxN, s.N);
+      c->nblk = std::max(c->nblk, s.dev.nblk);
+    }
+    void* p;
+    rc = dalloc(c, c->allocs, &p, n_systems * sizeof(SysDev));
+    if (rc == SCFT_OK) {
+      c->d_sys = (SysDev*)p;
+      std::vector<SysDev> rows;
+      for (auto& s : c->sys) rows.push_back(s.dev);
+      if (hipMemcpy(c->d_sys, rows.data(), rows.size() * sizeof(SysDev), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(SCFT_EDEVICE, "upload of system table failed");
+    }
+  }
+  if (rc == SCFT_OK) rc = build_programs(c);
+  // exp_nrH = forward FFT of the plate density (code/initialize.cpp:236-239)
+  for (int i = 0; i < n_systems && rc == SCFT_OK; i++) {
+    SysHost& s = c->sys[i];
+    if (hipMemcpyAsync(c->scratch[i], s.dev.rho_np, (size_t)s.M * 16, hipMemcpyDeviceToDevice, c->st) != hipSuccess)
+      rc = fail(SCFT_EDEVICE, "copy failed");
+    if (rc == SCFT_OK) rc = run_program(c, c->prog_fft_fwd[i], 0);
+    if (rc == SCFT_OK &&
+        hipMemcpyAsync(s.dev.exp_nrH, c->scratch[i], (size_t)s.M * 16, hipMemcpyDeviceToDevice, c->st) != hipSuccess)
+      rc = fail(SCFT_EDEVICE, "copy failed");
+  }
+  if (rc == SCFT_OK && hipStreamSynchronize(c->st) != hipSuccess) rc = fail(SCFT_EDEVICE, "set-up failed on the device");
+  if (rc != SCFT_OK) {
+    std::string msg = g_err;
+    scft_destroy(c);
+    g_err = msg;
+    return rc;
+  }
+  *out = c;
+  return SCFT_OK;
+}
+
+int scft_destroy(scft_ctx* c) {
+  if (!c) return SCFT_OK;
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  for (auto& s : c->sys)
+    for (void* p : s.allocs) (void)hipFree(p);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+  return SCFT_OK;
+}
+
+int scft_get_info(scft_ctx* c, int sys, scft_sysinfo* out) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  if (!out) return fail(SCFT_EINVAL, "null output");
+  *out = c->sys[sys].info;
+  return SCFT_OK;
+}
+
+int scft_set_fields(scft_ctx* c, int sys, const double* wpl, const double* wabp, const double* wabm) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  SysHost& s = c->sys[sys];
+  const size_t b = (size_t)s.M * 16;
+  if (wpl && (rc = copy_in(c, s.dev.wpl, wpl, b))) return rc;
+  if (wabp && (rc = copy_in(c, s.dev.wabp, wabp, b))) return rc;
+  if (wabm && (rc = copy_in(c, s.dev.wabm, wabm, b))) return rc;
+  return SCFT_OK;
+}
+
+int scft_get_fields(scft_ctx* c, int sys, double* wpl, double* wabp, double* wabm) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  SysHost& s = c->sys[sys];
+  const size_t b = (size_t)s.M * 16;
+  if (wpl && (rc = copy_out(c, wpl, s.dev.wpl, b))) return rc;
+  if (wabp && (rc = copy_out(c, wabp, s.dev.wabp, b))) return rc;
+  if (wabm && (rc = copy_out(c, wabm, s.dev.wabm, b))) return rc;
+  return SCFT_OK;
+}
+
+int scft_get_density(scft_ctx* c, int sys, double* rhoA, double* rhoB) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  SysHost& s = c->sys[sys];
+  const size_t b = (size_t)s.M * 16;
+  if (rhoA && (rc = copy_out(c, rhoA, s.dev.rhoA, b))) return rc;
+  if (rhoB && (rc = copy_out(c, rhoB, s.dev.rhoB, b))) return rc;
+  return SCFT_OK;
+}
+
+int scft_get_smeared(scft_ctx* c, int sys, double* smwa, double* smwb) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  SysHost& s = c->sys[sys];
+  const size_t b = (size_t)s.M * 16;
+  if (smwa && (rc = copy_out(c, smwa, s.dev.wa, b))) return rc;
+  if (smwb && (rc = copy_out(c, smwb, s.dev.wb, b))) return rc;
+  return SCFT_OK;
+}
+
+int scft_get_cavity(scft_ctx* c, int sys, double* rho) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  SysHost& s = c->sys[sys];
+  return copy_out(c, rho, s.dev.rho_np, (size_t)s.M * 16);
+}
+
+int scft_get_Q(scft_ctx* c, int sys, double* Q) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  SysHost& s = c->sys[sys];
+  double2 red[scft::RED_NSLOTS];
+  if ((rc = copy_out(c, red, s.dev.red, sizeof red))) return rc;
+  const cd Qs = cd(red[scft::RED_Q].x, red[scft::RED_Q].y) * (s.dV / s.info.V);
+  const cd Qt = Qs * std::exp(-double(s.N) * red[scft::RED_SHIFT].y);
+  Q[0] = Qt.real();
+  Q[1] = Qt.imag();
+  return SCFT_OK;
+}
+
+int scft_fft(scft_ctx* c, int sys, double* inout, int dir) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  if (dir != -1 && dir != 1) return fail(SCFT_EINVAL, "dir must be -1 or +1");
+  SysHost& s = c->sys[sys];
+  const size_t b = (size_t)s.M * 16;
+  HIPCK(hipMemcpyAsync(c->scratch[sys], inout, b, hipMemcpyHostToDevice, c->st));
+  rc = run_program(c, dir < 0 ? c->prog_fft_fwd[sys] : c->prog_fft_inv[sys], 0);
+  if (rc) return rc;
+  return copy_out(c, inout, c->scratch[sys], b);
+}
+
+int scft_calc_poly_density(scft_ctx* c) {
+  int rc = check_ctx(c, 0);
+  if (rc) return rc;
+  rc = stage_poly_density(c);
+  if (rc) return rc;
+  HIPCK(hipStreamSynchronize(c->st));
+  return SCFT_OK;
+}
+
+int scft_update_1s(scft_ctx* c) {
+  int rc = check_ctx(c, 0);
+  if (rc) return rc;
+  rc = stage_update(c, 1);
+  if (rc) return rc;
+  HIPCK(hipStreamSynchronize(c->st));
+  return SCFT_OK;
+}
+
+int scft_update_euler(scft_ctx* c) {
+  int rc = check_ctx(c, 0);
+  if (rc) return rc;
+  if (c->sys[0].p.do_CL) return fail(SCFT_EUNSUPPORTED, "complex-Langevin noise is not built yet");
+  rc = stage_update(c, 0);
+  if (rc) return rc;
+  HIPCK(hipStreamSynchronize(c->st));
+  return SCFT_OK;
+}
+
+int scft_calc_H(scft_ctx* c, double* H) {
+  int rc = check_ctx(c, 0);
+  if (rc) return rc;
+  if (!H) return fail(SCFT_EINVAL, "null output");
+  const int ns = (int)c->sys.size();
+  HIPCK(scft::launch_reduce(scft::RED_H, 0, c->d_sys, ns, c->nblk, c->st));
+  int nan = 0;
+  for (int i = 0; i < ns; i++) {
+    SysHost& s = c->sys[i];
+    double2 red[scft::RED_NSLOTS];
+    if ((rc = copy_out(c, red, s.dev.red, sizeof red))) return rc;
+    const cd field = cd(red[scft::RED_H].x, red[scft::RED_H].y) * s.dV;
+    const cd Qs = cd(red[scft::RED_Q].x, red[scft::RED_Q].y) * (s.dV / s.info.V);
+    cd logQ = std::log(Qs) - double(s.N) * red[scft::RED_SHIFT].y;
+    cd h = field;
+    if (s.info.nD > 0.0) h += -s.info.nD * logQ;
+    H[2 * i] = h.real();
+    H[2 * i + 1] = h.imag();
+    if (std::isnan(h.real()) || std::isnan(h.imag())) nan = 1;
+  }
+  if (nan) return fail(SCFT_ENAN, "H is NaN");
+  return SCFT_OK;
+}
+
+int scft_synchronize(scft_ctx* c) {
+  int rc = check_ctx(c, 0);
+  if (rc) return rc;
+  HIPCK(hipDeviceSynchronize());
+  return SCFT_OK;
+}
+
+int scft_time_kernel(scft_ctx* c, int which, int reps, double* avg_ms, double* bytes) {
+  int rc = check_ctx(c, 0);
+  if (rc) return rc;
+  if (reps < 1 || !avg_ms || !bytes) return fail(SCFT_EINVAL, "bad arguments");
+  const int ns = (int)c->sys.size();
+  double b = 0.0;
+  auto launch_one = [&]() -> int {
+    if (which == 0 || which == 1) {
+      const Program& p = c->prog_prop;
+      const size_t k = which == 0 ? 0 : 1;
+      if (k >= p.launches.size()) return fail(SCFT_EINVAL, "this grid has no separate z pass");
+      const Launch& L = p.launches[k];
+      HIPCK(scft::launch_pass(L.prec, L.mode, L.plane, L.d, L.njobs, L.gridx, L.tb, L.lds, 1, c->st));
+      return SCFT_OK;
+    }
+    if (which == 2) {
+      HIPCK(scft::launch_density(c->prec, c->d_sys, ns, c->maxM, c->st));
+      return SCFT_OK;
+    }
+    return stage_update(c, c->sys[0].p.update_scheme);
+  };
+  // algorithmic bytes of one launch
+  for (auto& s : c->sys) {
+    const double M = s.M, pe = (double)c->esz;
+    const int chains = (s.dev.qd != s.dev.q) ? 2 : 1;
+    const bool two_pass = s.nz > 1;
+    if (which == 0) b += chains * M * pe * (two_pass ? 2.0 : 3.0);
+    else if (which == 1) b += chains * M * pe * 3.0;
+    else if (which == 2) b += M * (2.0 * s.N * pe + 2 * 16.0 + 2 * 16.0);
+    else {
+      const double N = s.N;
+      b += M * (10.0 * (N - 1) * pe + 2 * pe + 2 * N * pe + 2 * 16.0 + 24 * 16.0 + 12 * 16.0);
+    }
+  }
+  if ((rc = launch_one())) return rc;  // warm
+  hipEvent_t e0, e1;
+  HIPCK(hipEventCreate(&e0));
+  HIPCK(hipEventCreate(&e1));
+  HIPCK(hipEventRecord(e0, c->st));
+  for (int r = 0; r < reps && rc == SCFT_OK; r++) rc = launch_one();
+  HIPCK(hipEventRecord(e1, c->st));
+  HIPCK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  *avg_ms = ms / reps;
+  *bytes = b;
+  return SCFT_OK;
+}
+
+int scft_iterate(scft_ctx* c, int n_iter) {
+  int rc = check_ctx(c, 0);
+  if (rc) return rc;
+  if (n_iter < 0) return fail(SCFT_EINVAL, "negative iteration count");
+  const int scheme = c->sys[0].p.update_scheme;
+  if (c->sys[0].p.do_CL) return fail(SCFT_EUNSUPPORTED, "complex-Langevin noise is not built yet");
+  if (!c->graph || c->graph_scheme != scheme) {
+    // the first iteration runs eagerly (sets kernel attributes), the
+    // second is captured
+    if (n_iter == 0) return SCFT_OK;
+    if ((rc = stage_update(c, scheme))) return rc;
+    n_iter--;
+    if (c->graph) {
+      (void)hipGraphExecDestroy(c->graph);
+      c->graph = nullptr;
+    }
+    hipGraph_t g;
+    HIPCK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    rc = stage_update(c, scheme);
+    hipError_t e = hipStreamEndCapture(c->st, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SCFT_EDEVICE, "graph capture failed");
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(SCFT_EDEVICE, "graph instantiation failed");
+    c->graph_scheme = scheme;
+  }
+  for (int it = 0; it < n_iter; it++) HIPCK(hipGraphLaunch(c->graph, c->st));
+  HIPCK(hipStreamSynchronize(c->st));
+  return SCFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// Device-side descriptors shared by the kernels (scft_kernels.hip) and the
+// host context (scft_api.hip).  All tables live in device memory and are
+// built once per context; per-call variation (the contour step) is a scalar
+// kernel argument.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lds_fft.hpp"
+
+namespace scft {
+
+enum { MODE_CONV = 0, MODE_FWD = 1, MODE_INV = 2 };
+
+// One array of one system for an FFT-style pass.
+struct PassDesc {
+  const void* in;
+  void* out;
+  long in_step, out_step;   // element offset per contour step
+  int in_soff, out_soff;    // in = in + (step + in_soff) * in_step
+  const void* post_a;       // optional pointwise multiplier on the last store
+  const void* post_b;
+  int post_switch;          // post = step < post_switch ? post_a : post_b
+  int steps_active;         // job runs while step < steps_active (0: always)
+  const void* g0;           // 1D multiplier of the pass axis (conv mode)
+  const void* g1;           // plane pass: multiplier along y
+  const void* tw0;          // twiddles of the pass axis, exp(-2 pi i m / n)
+  const void* tw1;
+  Plan p0, p1;
+  int nx, ny, nz;
+  int axis;                 // line pass axis: 0 x, 1 y, 2 z
+  int tl;                   // lines per tile (line pass)
+  int ntiles;
+  int last;                 // this pass applies post and scale
+  double scale;
+};
+
+// Per-system state for the pointwise kernels.
+struct SysDev {
+  int M, N, Na, nx, ny, nz;
+  int dim, neg_ab, do_cl, np_chem;
+  double nD, V, dV, rho0, kappa, chi_ab, lam_imag, lam_real;
+  double fA;                // Na / N, weights of the gauge shift
+  double2 *wpl, *wabp, *wabm;
+  double2 *rhoA, *rhoB;
+  double2 *wa, *wb;         // assembled fields, smeared in place -> smw
+  double2 *rho_np;          // plate density, real space
+  double2 *exp_nrH;         // its forward transform
+  double *hhat_k;           // smearing kernel in k-space
+  double *Xk;               // phiD rho0 N h^2 (gaa + 2 gab + gbb) (+ graft term)
+  double *X2k;              // phiD rho0 N h^2 (gaa - gbb), negative-chi branch
+  void *q, *qd;             // N * M propagators, propagator precision
+  void *expA, *expB;        // exp(-(smw - shift)), propagator precision
+  double2 *etap, *etam;     // noise (k-space)
+  double2 *red;             // scalar slots, see RED_*
+  double2 *partials;        // reduction scratch, nblk per slot
+  int nblk;                 // partial blocks of a full-grid reduction
+};
+
+enum {
+  RED_Q = 0,       // sum of q[N-1] (then Q = sum * dV / V, shifted gauge)
+  RED_H = 1,       // sum of the field part of H
+  RED_SHIFT = 2,   // sum of Re(fA smwa + (1 - fA) smwb); .y = shift c
+  RED_RHO = 3,     // sum of rhoA + rhoB (segment count check)
+  RED_NSLOTS = 4
+};
+
+}  // namespace scft

@@ -1,0 +1,544 @@
+// HIP kernels of the SCFT hot path for gfx950 and their launch wrappers.
+//
+// FFT-style passes (k_plane, k_line) run the in-LDS Stockham engine of
+// lds_fft.hpp over tiles described by a device table of PassDesc, one table
+// row per (array, system); blockIdx.y selects the row, blockIdx.x the tile.
+// The propagator step q_{s+1} = exp(-w) IFFT[Phi(k) FFT q_s]
+// (code/diblock_discrete.cpp:44-54) uses that Phi(k) = exp(-k^2/6) is a
+// product of 1D Gaussians: the 3D convolution is done as 1D
+// forward-FFT / multiply / inverse-FFT along x and y inside one LDS-resident
+// xy plane, then along z in strided pencil tiles whose store applies exp(-w).
+// Pointwise kernels take a device table of SysDev, blockIdx.y = system.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "scft_dev.hpp"
+
+namespace scft {
+
+template <typename R>
+__device__ __forceinline__ cpx<R> ld(const void* base, long i) {
+  return reinterpret_cast<const cpx<R>*>(base)[i];
+}
+
+__device__ __forceinline__ double2 dadd(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 dmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 dscale(double2 a, double s) {
+  return make_double2(a.x * s, a.y * s);
+}
+__device__ __forceinline__ double2 ddiv(double2 a, double2 b) {
+  // a / b with the scaling used by std::complex for well-ranged values
+  const double den = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
+}
+__device__ __forceinline__ double2 dexp(double2 a) {
+  double s, c;
+  sincos(a.y, &s, &c);
+  const double e = exp(a.x);
+  return make_double2(e * c, e * s);
+}
+__device__ __forceinline__ double2 mul_i(double2 a) { return make_double2(-a.y, a.x); }
+
+// ---------------------------------------------------------------------------
+// Plane pass: x and y lines of one xy plane in LDS.
+// ---------------------------------------------------------------------------
+template <typename R, int MODE, int TB, int MAXV>
+__global__ __launch_bounds__(TB) void k_plane(const PassDesc* __restrict__ descs, int step) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  const PassDesc& d = descs[blockIdx.y];
+  if (d.steps_active && step >= d.steps_active) return;
+  const int z = blockIdx.x;
+  if (z >= d.nz) return;
+  const int tid = threadIdx.x;
+  const int nx = d.nx, ny = d.ny, nxy = nx * ny;
+  const long poff = (long)z * nxy;
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step + d.in_soff) * d.in_step + poff;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step + d.out_soff) * d.out_step + poff;
+  for (int i = tid; i < nxy; i += TB) lds[i] = in[i];
+  __syncthreads();
+  const cpx<R>* tw0 = reinterpret_cast<const cpx<R>*>(d.tw0);
+  const cpx<R>* tw1 = reinterpret_cast<const cpx<R>*>(d.tw1);
+  const R* g0 = reinterpret_cast<const R*>(d.g0);
+  const R* g1 = reinterpret_cast<const R*>(d.g1);
+  // x lines: B = ny lines, line-major (ls = nx, ps = 1)
+  if (MODE == MODE_CONV) {
+    lds_fft<R, -1, MAXV>(d.p0, lds, ny, nx, 1, tw0, g0, tid, TB);
+    lds_fft<R, +1, MAXV>(d.p0, lds, ny, nx, 1, tw0, nullptr, tid, TB);
+  } else if (MODE == MODE_FWD) {
+    lds_fft<R, -1, MAXV>(d.p0, lds, ny, nx, 1, tw0, nullptr, tid, TB);
+  } else {
+    lds_fft<R, +1, MAXV>(d.p0, lds, ny, nx, 1, tw0, nullptr, tid, TB);
+  }
+  // y lines: B = nx lines, point-major (ls = 1, ps = nx)
+  if (ny > 1) {
+    if (MODE == MODE_CONV) {
+      lds_fft<R, -1, MAXV>(d.p1, lds, nx, 1, nx, tw1, g1, tid, TB);
+      lds_fft<R, +1, MAXV>(d.p1, lds, nx, 1, nx, tw1, nullptr, tid, TB);
+    } else if (MODE == MODE_FWD) {
+      lds_fft<R, -1, MAXV>(d.p1, lds, nx, 1, nx, tw1, nullptr, tid, TB);
+    } else {
+      lds_fft<R, +1, MAXV>(d.p1, lds, nx, 1, nx, tw1, nullptr, tid, TB);
+    }
+  }
+  const cpx<R>* post = nullptr;
+  R scale = R(1);
+  if (d.last) {
+    const void* pp = (step < d.post_switch) ? d.post_a : d.post_b;
+    if (pp) post = reinterpret_cast<const cpx<R>*>(pp) + poff;
+    scale = R(d.scale);
+  }
+  for (int i = tid; i < nxy; i += TB) {
+    cpx<R> o = lds[i];
+    if (post) o = cmul(o, post[i]);
+    if (scale != R(1)) o = cscale(o, scale);
+    out[i] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Line pass along one axis for a tile of `tl` lines.
+//   axis 0: lines are contiguous rows (line-major tile);
+//   axis 1: line l = x + nx z, points strided by nx;
+//   axis 2: line l = x + nx y, points strided by nx ny.
+// For axes 1 and 2 consecutive lines are consecutive in memory, so the
+// point-major tile is loaded with coalesced row segments.
+// ---------------------------------------------------------------------------
+template <typename R, int MODE, int TB, int MAXV>
+__global__ __launch_bounds__(TB) void k_line(const PassDesc* __restrict__ descs, int step) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  const PassDesc& d = descs[blockIdx.y];
+  if (d.steps_active && step >= d.steps_active) return;
+  if ((int)blockIdx.x >= d.ntiles) return;
+  const int tid = threadIdx.x;
+  const int axis = d.axis, nx = d.nx, ny = d.ny, nz = d.nz;
+  const int n = d.p0.n;
+  const long M = (long)nx * ny * nz;
+  const int nlines = (int)(M / n);
+  const int tl = d.tl;
+  const int l0 = blockIdx.x * tl;
+  const long S = axis == 0 ? 1 : (axis == 1 ? nx : (long)nx * ny);
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step + d.in_soff) * d.in_step;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step + d.out_soff) * d.out_step;
+  const cpx<R>* tw = reinterpret_cast<const cpx<R>*>(d.tw0);
+  const R* g = reinterpret_cast<const R*>(d.g0);
+  const int ntot = tl * n;
+  int ls, ps;
+  if (axis == 0) {
+    ls = n;
+    ps = 1;
+    for (int idx = tid; idx < ntot; idx += TB) {
+      const int l = idx / n, i = idx - l * n;
+      const int gl = l0 + l;
+      lds[idx] = (gl < nlines) ? in[(long)gl * n + i] : mk(R(0), R(0));
+    }
+  } else {
+    ls = 1;
+    ps = tl;
+    for (int idx = tid; idx < ntot; idx += TB) {
+      const int i = idx / tl, l = idx - i * tl;
+      const int gl = l0 + l;
+      long base;
+      if (axis == 1) base = (long)(gl % nx) + (long)nx * ny * (gl / nx);
+      else base = gl;
+      lds[idx] = (gl < nlines) ? in[base + i * S] : mk(R(0), R(0));
+    }
+  }
+  __syncthreads();
+  if (MODE == MODE_CONV) {
+    lds_fft<R, -1, MAXV>(d.p0, lds, tl, ls, ps, tw, g, tid, TB);
+    lds_fft<R, +1, MAXV>(d.p0, lds, tl, ls, ps, tw, nullptr, tid, TB);
+  } else if (MODE == MODE_FWD) {
+    lds_fft<R, -1, MAXV>(d.p0, lds, tl, ls, ps, tw, nullptr, tid, TB);
+  } else {
+    lds_fft<R, +1, MAXV>(d.p0, lds, tl, ls, ps, tw, nullptr, tid, TB);
+  }
+  const cpx<R>* post = nullptr;
+  R scale = R(1);
+  if (d.last) {
+    const void* pp = (step < d.post_switch) ? d.post_a : d.post_b;
+    if (pp) post = reinterpret_cast<const cpx<R>*>(pp);
+    scale = R(d.scale);
+  }
+  for (int idx = tid; idx < ntot; idx += TB) {
+    int l, i;
+    if (axis == 0) {
+      l = idx / n;
+      i = idx - l * n;
+    } else {
+      i = idx / tl;
+      l = idx - i * tl;
+    }
+    const int gl = l0 + l;
+    if (gl >= nlines) continue;
+    long base;
+    if (axis == 0) base = (long)gl * n;
+    else if (axis == 1) base = (long)(gl % nx) + (long)nx * ny * (gl / nx);
+    else base = gl;
+    const long gi = base + i * S;
+    cpx<R> o = lds[idx];
+    if (post) o = cmul(o, post[gi]);
+    if (scale != R(1)) o = cscale(o, scale);
+    out[gi] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Field assembly, code/calc_poly_density.cpp:45-50 (AB pair):
+//   wa = i w+ + (i wAB+ - wAB-)      [negative chi: i wAB- - wAB+]
+//   wb = i w+ + (i wAB+ + wAB-)      [negative chi: -i wAB- - wAB+]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_assemble(const SysDev* __restrict__ sys) {
+  const SysDev& s = sys[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.M) return;
+  const double2 wp = s.wpl[i], p = s.wabp[i], m = s.wabm[i];
+  const double2 iwp = mul_i(wp);
+  double2 a, b;
+  if (!s.neg_ab) {
+    const double2 ip = mul_i(p);
+    a = make_double2(iwp.x + (ip.x - m.x), iwp.y + (ip.y - m.y));
+    b = make_double2(iwp.x + (ip.x + m.x), iwp.y + (ip.y + m.y));
+  } else {
+    const double2 im = mul_i(m);
+    a = make_double2(iwp.x + (im.x - p.x), iwp.y + (im.y - p.y));
+    b = make_double2(iwp.x + (-im.x - p.x), iwp.y + (-im.y - p.y));
+  }
+  s.wa[i] = a;
+  s.wb[i] = b;
+}
+
+// exp(-(smw - c)) into the propagator precision, and the chain ends
+// q_0 = exp(-wA) (Na != 0) / q-dagger_0 = exp(-wB) (Na != N),
+// code/diblock_discrete.cpp:19-29.
+template <typename R>
+__global__ __launch_bounds__(256) void k_exp_prop(const SysDev* __restrict__ sys) {
+  const SysDev& s = sys[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.M) return;
+  const double c = s.red[RED_SHIFT].y;
+  const double2 a = s.wa[i], b = s.wb[i];
+  const double2 ea = dexp(make_double2(-(a.x - c), -a.y));
+  const double2 eb = dexp(make_double2(-(b.x - c), -b.y));
+  cpx<R> ca = mk(R(ea.x), R(ea.y)), cb = mk(R(eb.x), R(eb.y));
+  reinterpret_cast<cpx<R>*>(s.expA)[i] = ca;
+  reinterpret_cast<cpx<R>*>(s.expB)[i] = cb;
+  reinterpret_cast<cpx<R>*>(s.q)[i] = (s.Na != 0) ? ca : cb;
+  if (s.qd != s.q) reinterpret_cast<cpx<R>*>(s.qd)[i] = (s.Na != s.N) ? cb : ca;
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic two-stage FP64 reductions.  Stage 1: block b of system s
+// sums points b*256+t, stride nblk*256, then a fixed LDS tree.  Stage 2: one
+// block sums the nblk partials in a fixed order into s.red[slot].
+// ---------------------------------------------------------------------------
+template <int WHAT, typename R>
+__device__ __forceinline__ double2 red_term(const SysDev& s, long i) {
+  if constexpr (WHAT == RED_Q) {
+    const cpx<R> v = ld<R>(s.q, (long)(s.N - 1) * s.M + i);
+    return make_double2((double)v.x, (double)v.y);
+  } else if constexpr (WHAT == RED_H) {
+    // wpl_part + wab_part integrands, code/calc_h.cpp:70-73, :97-98
+    const double2 w = s.wpl[i];
+    double2 t = make_double2(0.0, 0.0);
+    if (s.kappa > 0.0) {
+      const double2 w2 = dmul(w, w);
+      t = dscale(w2, s.rho0 / s.kappa / 2.0);
+    }
+    // rho0 (1 - rho_np), rho_np stored complex
+    const double2 c = make_double2(s.rho0 * (1.0 - s.rho_np[i].x), -s.rho0 * s.rho_np[i].y);
+    const double2 icw = mul_i(dmul(c, w));
+    t = make_double2(t.x - icw.x, t.y - icw.y);
+    if (s.chi_ab != 0.0) {
+      const double2 p = s.wabp[i], m = s.wabm[i];
+      const double2 pp = dmul(p, p), mm = dmul(m, m);
+      const double sg = s.neg_ab ? -1.0 : 1.0;
+      t = dadd(t, dscale(dadd(pp, mm), s.rho0 / s.chi_ab * sg));
+    }
+    return t;
+  } else if constexpr (WHAT == RED_SHIFT) {
+    return make_double2(s.fA * s.wa[i].x + (1.0 - s.fA) * s.wb[i].x, 0.0);
+  } else {
+    return dadd(s.rhoA[i], s.rhoB[i]);
+  }
+}
+
+template <int WHAT, typename R>
+__global__ __launch_bounds__(256) void k_reduce1(const SysDev* __restrict__ sys) {
+  const SysDev& s = sys[blockIdx.y];
+  __shared__ double2 sh[256];
+  double2 acc = make_double2(0.0, 0.0);
+  if ((int)blockIdx.x < s.nblk) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < s.M; i += (long)s.nblk * 256)
+      acc = dadd(acc, red_term<WHAT, R>(s, i));
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = dadd(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && (int)blockIdx.x < s.nblk) s.partials[WHAT * s.nblk + blockIdx.x] = sh[0];
+}
+
+template <int WHAT>
+__global__ __launch_bounds__(256) void k_reduce2(const SysDev* __restrict__ sys) {
+  const SysDev& s = sys[blockIdx.y];
+  __shared__ double2 sh[256];
+  double2 acc = make_double2(0.0, 0.0);
+  for (int b = threadIdx.x; b < s.nblk; b += 256) acc = dadd(acc, s.partials[WHAT * s.nblk + b]);
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = dadd(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double2 r = sh[0];
+    if (WHAT == RED_SHIFT) r = make_double2(r.x, r.x / (double)s.M);  // .y = mean
+    s.red[WHAT] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Density, code/calc_poly_density.cpp:167-191:
+//   rhoA = nD/(Q V) exp(smwA) sum_{j<Na} q_j qdag_{N-1-j}, rhoB over j >= Na,
+// accumulated in FP64 whatever the propagator precision.  With the gauge
+// shift c both Q and the q's carry exp(N c); exp(smw - c) restores rho.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_density(const SysDev* __restrict__ sys) {
+  const SysDev& s = sys[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.M) return;
+  const double2 Qs = dscale(s.red[RED_Q], s.dV / s.V);
+  const double2 factor = ddiv(make_double2(s.nD, 0.0), dscale(Qs, s.V));
+  const double c = s.red[RED_SHIFT].y;
+  double2 accA = make_double2(0.0, 0.0), accB = make_double2(0.0, 0.0);
+  const cpx<R>* q = reinterpret_cast<const cpx<R>*>(s.q);
+  const cpx<R>* qd = reinterpret_cast<const cpx<R>*>(s.qd);
+  const long M = s.M;
+  for (int j = 0; j < s.N; j++) {
+    const cpx<R> a = q[(long)j * M + i];
+    const cpx<R> b = qd[(long)(s.N - 1 - j) * M + i];
+    const double2 p = make_double2((double)a.x * b.x - (double)a.y * b.y,
+                                   (double)a.x * b.y + (double)a.y * b.x);
+    if (s.Na > 0 && j < s.Na) accA = dadd(accA, p);
+    else accB = dadd(accB, p);
+  }
+  const double2 wa = s.wa[i], wb = s.wb[i];
+  const double2 ea = dexp(make_double2(wa.x - c, wa.y));
+  const double2 eb = dexp(make_double2(wb.x - c, wb.y));
+  s.rhoA[i] = dmul(accA, dmul(ea, factor));
+  s.rhoB[i] = dmul(accB, dmul(eb, factor));
+}
+
+// ---------------------------------------------------------------------------
+// 1S semi-implicit update in k-space, code/1s_update.cpp:57-160 (w+ and the
+// AB pair).  The k = 0 delta of the w+ force is at linear index 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_update_1s(const SysDev* __restrict__ sys) {
+  const SysDev& s = sys[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.M) return;
+  const double rho0 = s.rho0, li = s.lam_imag, lr = s.lam_real;
+  const double h = s.hhat_k[i];
+  const double2 rA = s.rhoA[i], rB = s.rhoB[i];
+  const double2 rsum = dadd(rA, rB);
+  const double2 enr = s.exp_nrH[i];
+  const double evar = (i == 0) ? 1.0 : 0.0;
+  {
+    // F = i rho0 (exp_nrH - evar) + i h (rhoA + rhoB)
+    const double2 in = make_double2(rho0 * (enr.x - evar) + h * rsum.x, rho0 * enr.y + h * rsum.y);
+    const double2 F = mul_i(in);
+    const double A = (s.kappa <= 0.0 ? 0.0 : rho0 / s.kappa) + s.Xk[i];
+    const double B = s.kappa > 0.0 ? A - rho0 / s.kappa : 0.0;
+    const double2 w = s.wpl[i];
+    double2 num = make_double2(w.x - li * (F.x - B * w.x), w.y - li * (F.y - B * w.y));
+    if (s.do_cl) num = dadd(num, s.etap[i]);
+    s.wpl[i] = dscale(num, 1.0 / (1.0 + li * A));
+  }
+  if (s.chi_ab != 0.0) {
+    const double2 wp = s.wabp[i], wm = s.wabm[i];
+    const double c2 = 2.0 * rho0 / s.chi_ab;
+    if (!s.neg_ab) {
+      double2 F = mul_i(dscale(rsum, h));
+      if (s.np_chem == 1) F = make_double2(F.x - rho0 * enr.x, F.y - rho0 * enr.y);
+      const double A = c2 + s.Xk[i];
+      const double B = A - c2;
+      double2 num = make_double2(wp.x - li * (F.x - B * wp.x), wp.y - li * (F.y - B * wp.y));
+      if (s.do_cl) num = dadd(num, s.etap[s.M + i]);
+      s.wabp[i] = dscale(num, 1.0 / (1.0 + li * A));
+      double2 Fm = dscale(make_double2(rB.x - rA.x, rB.y - rA.y), h);
+      if (s.np_chem == 1) Fm = dadd(Fm, mul_i(dscale(enr, rho0)));
+      double2 numm = make_double2(wm.x - lr * Fm.x, wm.y - lr * Fm.y);
+      if (s.do_cl) numm = dadd(numm, s.etam[i]);
+      s.wabm[i] = dscale(numm, 1.0 / (1.0 + lr * c2));
+    } else {
+      const double2 F = dscale(rsum, -h);
+      s.wabp[i] = dscale(make_double2(wp.x - lr * F.x, wp.y - lr * F.y), 1.0 / (1.0 + lr * c2));
+      double2 Fm = mul_i(dscale(make_double2(rA.x - rB.x, rA.y - rB.y), h));
+      if (s.np_chem == 1) Fm = make_double2(Fm.x - rho0 * enr.x, Fm.y - rho0 * enr.y);
+      const double A = c2 + s.X2k[i];
+      const double B = A - c2;
+      double2 num = make_double2(wm.x - li * (Fm.x - B * wm.x), wm.y - li * (Fm.y - B * wm.y));
+      if (s.do_cl) num = dadd(num, s.etap[s.M + i]);
+      s.wabm[i] = dscale(num, 1.0 / (1.0 + li * A));
+    }
+  } else {
+    s.wabp[i] = make_double2(0.0, 0.0);
+    s.wabm[i] = make_double2(0.0, 0.0);
+  }
+}
+
+// Explicit Euler / complex-Langevin step in k-space,
+// code/Euler_update.cpp:52-99 (w+ and the AB pair).
+__global__ __launch_bounds__(256) void k_update_euler(const SysDev* __restrict__ sys) {
+  const SysDev& s = sys[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.M) return;
+  const double rho0 = s.rho0, li = s.lam_imag, lr = s.lam_real;
+  const double h = s.hhat_k[i];
+  const double2 rA = s.rhoA[i], rB = s.rhoB[i];
+  const double2 enr = s.exp_nrH[i];
+  const double evar = (i == 0) ? 1.0 : 0.0;
+  {
+    const double2 w = s.wpl[i];
+    double2 F = s.kappa <= 0.0 ? make_double2(0.0, 0.0) : dscale(w, rho0 / s.kappa);
+    const double2 in = make_double2(rho0 * (enr.x - evar) + h * (rA.x + rB.x),
+                                    rho0 * enr.y + h * (rA.y + rB.y));
+    F = dadd(F, mul_i(in));
+    double2 num = make_double2(w.x - li * F.x, w.y - li * F.y);
+    if (s.do_cl) num = dadd(num, s.etap[i]);
+    s.wpl[i] = num;
+  }
+  if (s.chi_ab != 0.0) {
+    const double2 wp = s.wabp[i], wm = s.wabm[i];
+    const double c2 = 2.0 * rho0 / s.chi_ab;
+    // A segments and B segments enter with sp (AB+) and sa, sb (AB-)
+    double2 Fp, Fm;
+    if (!s.neg_ab) {
+      Fp = dadd(dscale(wp, c2), mul_i(dscale(dadd(rA, rB), h)));
+      Fm = dadd(dscale(wm, c2), dscale(make_double2(rB.x - rA.x, rB.y - rA.y), h));
+    } else {
+      Fp = dadd(dscale(wp, c2), dscale(dadd(rA, rB), -h));
+      Fm = dadd(dscale(wm, c2), mul_i(dscale(make_double2(rA.x - rB.x, rA.y - rB.y), h)));
+    }
+    double2 np_ = make_double2(wp.x - li * Fp.x, wp.y - li * Fp.y);
+    double2 nm = make_double2(wm.x - lr * Fm.x, wm.y - lr * Fm.y);
+    if (s.do_cl) {
+      np_ = dadd(np_, s.etap[s.M + i]);
+      nm = dadd(nm, s.etam[i]);
+    }
+    s.wabp[i] = np_;
+    s.wabm[i] = nm;
+  } else {
+    s.wabp[i] = make_double2(0.0, 0.0);
+    s.wabm[i] = make_double2(0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+template <typename R, int MODE, int TB, bool PLANE>
+static void set_lds_attr() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  if (PLANE)
+    (void)hipFuncSetAttribute((const void*)k_plane<R, MODE, TB, 32>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  else
+    (void)hipFuncSetAttribute((const void*)k_line<R, MODE, TB, 32>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+template <typename R, int MODE>
+static hipError_t launch_pass_t(bool plane, const PassDesc* d, int njobs, int gridx, int tb,
+                                size_t lds, int step, hipStream_t st) {
+  dim3 grid(gridx, njobs);
+#define SCFT_LAUNCH(TB)                                                                  \
+  do {                                                                                   \
+    if (plane) {                                                                         \
+      if (lds > 64 * 1024) set_lds_attr<R, MODE, TB, true>();                             \
+      hipLaunchKernelGGL((k_plane<R, MODE, TB, 32>), grid, dim3(TB), lds, st, d, step);  \
+    } else {                                                                             \
+      if (lds > 64 * 1024) set_lds_attr<R, MODE, TB, false>();                            \
+      hipLaunchKernelGGL((k_line<R, MODE, TB, 32>), grid, dim3(TB), lds, st, d, step);   \
+    }                                                                                    \
+  } while (0)
+  if (tb <= 256) SCFT_LAUNCH(256);
+  else if (tb <= 512) SCFT_LAUNCH(512);
+  else SCFT_LAUNCH(1024);
+#undef SCFT_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_pass(int prec, int mode, bool plane, const PassDesc* d, int njobs, int gridx,
+                       int tb, size_t lds, int step, hipStream_t st) {
+  if (prec == 0) {
+    if (mode == MODE_CONV) return launch_pass_t<double, MODE_CONV>(plane, d, njobs, gridx, tb, lds, step, st);
+    if (mode == MODE_FWD) return launch_pass_t<double, MODE_FWD>(plane, d, njobs, gridx, tb, lds, step, st);
+    return launch_pass_t<double, MODE_INV>(plane, d, njobs, gridx, tb, lds, step, st);
+  }
+  if (mode == MODE_CONV) return launch_pass_t<float, MODE_CONV>(plane, d, njobs, gridx, tb, lds, step, st);
+  if (mode == MODE_FWD) return launch_pass_t<float, MODE_FWD>(plane, d, njobs, gridx, tb, lds, step, st);
+  return launch_pass_t<float, MODE_INV>(plane, d, njobs, gridx, tb, lds, step, st);
+}
+
+static inline dim3 pgrid(int maxM, int nsys) { return dim3((maxM + 255) / 256, nsys); }
+
+hipError_t launch_assemble(const SysDev* s, int nsys, int maxM, hipStream_t st) {
+  hipLaunchKernelGGL(k_assemble, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_exp_prop(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st) {
+  if (prec == 0) hipLaunchKernelGGL(k_exp_prop<double>, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  else hipLaunchKernelGGL(k_exp_prop<float>, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_density(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st) {
+  if (prec == 0) hipLaunchKernelGGL(k_density<double>, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  else hipLaunchKernelGGL(k_density<float>, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_update(int scheme, const SysDev* s, int nsys, int maxM, hipStream_t st) {
+  if (scheme == 1) hipLaunchKernelGGL(k_update_1s, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  else hipLaunchKernelGGL(k_update_euler, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(int what, int prec, const SysDev* s, int nsys, int nblk, hipStream_t st) {
+  dim3 g1(nblk, nsys), g2(1, nsys);
+  switch (what) {
+    case RED_Q:
+      if (prec == 0) hipLaunchKernelGGL((k_reduce1<RED_Q, double>), g1, dim3(256), 0, st, s);
+      else hipLaunchKernelGGL((k_reduce1<RED_Q, float>), g1, dim3(256), 0, st, s);
+      hipLaunchKernelGGL(k_reduce2<RED_Q>, g2, dim3(256), 0, st, s);
+      break;
+    case RED_H:
+      hipLaunchKernelGGL((k_reduce1<RED_H, double>), g1, dim3(256), 0, st, s);
+      hipLaunchKernelGGL(k_reduce2<RED_H>, g2, dim3(256), 0, st, s);
+      break;
+    case RED_SHIFT:
+      hipLaunchKernelGGL((k_reduce1<RED_SHIFT, double>), g1, dim3(256), 0, st, s);
+      hipLaunchKernelGGL(k_reduce2<RED_SHIFT>, g2, dim3(256), 0, st, s);
+      break;
+    default:
+      hipLaunchKernelGGL((k_reduce1<RED_RHO, double>), g1, dim3(256), 0, st, s);
+      hipLaunchKernelGGL(k_reduce2<RED_RHO>, g2, dim3(256), 0, st, s);
+      break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace scft

@@ -5,10 +5,11 @@ The library is the product: HIP kernels for gfx950 behind a plain C ABI.  This
 module only moves numpy arrays across that ABI.  It raises if the library is
 missing; there is no CPU fallback.
 
-Parameter mapping from polymer units (Rg, chi N, C, kappa N) to the
+Parameter mapping from polymer units (Rg, chi N, C, kappa N, dt) to the
 reference's b units (code/initialize.cpp:55 sets Rg^2 = (N-1)/6 and
 code/initialize.cpp:403 uses exp(-k^2/6) bonds):
-    Rg = sqrt((N-1)/6) b,  chi = chiN/N,  kappa = kappaN/N,  rho0 = C N / Rg^3.
+    Rg = sqrt((N-1)/6) b,  chi = chiN/N,  kappa = kappaN/N,  rho0 = C N / Rg^3,
+    lam_imag = lam_real = dt / (rho0 N).
 """
 from __future__ import annotations
 
@@ -159,8 +160,11 @@ class Workload:
         for d in range(3):
             p.Nx[d] = self.grid[d] if d < len(self.grid) else 1
             p.L[d] = self.box_rg[d] * rg if d < len(self.box_rg) else 1.0
-        p.lam_imag = self.dt
-        p.lam_real = self.dt
+        # polymer-unit step dt: with w~ = N w, r~ = r / Rg and H = C H~, the
+        # force obeys dH/dw = rho0 dH~/dw~, so w~ <- w~ - dt dH~/dw~ is the
+        # reference's w <- w - lam dH/dw with lam = dt / (rho0 N)
+        p.lam_imag = self.dt / (p.rho0 * self.N)
+        p.lam_real = self.dt / (p.rho0 * self.N)
         p.itermax = self.itermax
         p.print_freq = self.print_freq
         p.sample_freq, p.sample_wait = 50, 0
@@ -265,6 +269,8 @@ def load_library(path: str | None = None):
         "scft_update_euler": (ctypes.c_int, [vp]),
         "scft_calc_H": (ctypes.c_int, [vp, dp]),
         "scft_iterate": (ctypes.c_int, [vp, ctypes.c_int]),
+        "scft_synchronize": (ctypes.c_int, [vp]),
+        "scft_time_kernel": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, dp, dp]),
         "scft_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -386,6 +392,17 @@ class Context:
 
     def iterate(self, n: int):
         self._check(self.lib.scft_iterate(self.h, n), "scft_iterate")
+
+    def synchronize(self):
+        self._check(self.lib.scft_synchronize(self.h), "scft_synchronize")
+
+    def time_kernel(self, which: int, reps: int = 20):
+        """(average ms, algorithmic bytes) of one launch, see scft_time_kernel."""
+        ms = ctypes.c_double()
+        b = ctypes.c_double()
+        self._check(self.lib.scft_time_kernel(self.h, which, reps, ctypes.byref(ms), ctypes.byref(b)),
+                    "scft_time_kernel")
+        return ms.value, b.value
 
 
 def read_input(path: str, dim: int) -> ScftParams:

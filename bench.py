@@ -1,0 +1,173 @@
+"""Benchmark of the SCFT hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+A step is one SCFT iteration of the production system (64 x 64 x 128 grid,
+box 4 x 4 x 8 Rg, AB diblock N = 100, f_A = 0.5, chi N = 20, C = 50,
+kappa N = 50, two square plates at d = 0.30): the 1S semi-implicit field
+update (5 forward + 3 inverse 3D FFTs and the k-space update) followed by the
+density evaluation (field assembly, smearing, 2 x 99 propagator steps, Q and
+the q.q-dagger sum).  Inputs are resident in HBM before timing starts; K
+iterations run as replays of one captured HIP graph, bracketed by
+hipDeviceSynchronize (scft_synchronize).  torch is not imported: the library
+brings its own HIP runtime and is the only GPU client of the process.
+
+`value` is FP32-propagator iterations/s (complex64 propagator and FFTs,
+complex128 fields, FP64 Q, log Q and H).  The plain complex128 path is timed
+the same way and reported beside it, with |dH/H| between the two after the
+same iterations from the same start.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "aligned-nanoplate-separation-and-rotation_amd"))
+
+import numpy as np  # noqa: E402
+
+import scft_amd  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "production-64x64x128-N100"
+METRIC = "SCFT iter/s at 64x64x128 N=100: FP32-batched vs FP64 plain, %HBM roofline"
+
+
+def b_iter(M, N, b, bf):
+    """Algorithmic bytes of one SCFT iteration (reference dataflow: q and
+    q-dagger stored, each 3D FFT one read + one write)."""
+    return M * (10 * (N - 1) * b + 2 * b + 2 * N * b + 2 * bf + 24 * bf + 12 * bf)
+
+
+def run_path(p, fields, prec, warmup, steps):
+    # torch is not imported: the library is the only HIP client of this process
+    ctx = scft_amd.Context(p, device=0, precision=prec)
+    ctx.set_fields(*fields)
+    ctx.calc_poly_density()
+    ctx.iterate(max(warmup, 1))  # includes the eager iteration and graph capture
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.iterate(steps)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    H = ctx.calc_H()[0]
+    return ctx, dt, H
+
+
+def cpu_baseline(p, fields, budget_s=20.0):
+    """FP64 oracle (reference dataflow) on the host cores, bounded sample."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    pyoracle.set_threads(threads)
+    o = pyoracle.Oracle(p)
+    o.set_fields(*fields)
+    o.calc_poly_density()
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.update_1s()
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 50:
+            break
+    dt = time.perf_counter() - t0
+    o.close()
+    return {
+        "value": n / dt,
+        "unit": "iter/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} 1S iterations of the 64x64x128 N=100 system (FP64 oracle, OpenMP, "
+                  f"{dt:.1f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
+              "(independent systems shard across GPUs without exchange; no multi-GPU run here).")
+        sys.exit(2)
+
+    w = scft_amd.config(WORKLOAD)
+    p = w.params()
+    fields = scft_amd.seeded_fields(p, w.seed, period_rg=4.0, amp=1.0, sigma=0.01)
+    M, N = p.M, p.N
+
+    ctx32, dt32, H32 = run_path(p, fields, scft_amd.PREC_FP32, args.warmup, args.steps)
+    it32 = args.steps / dt32
+    # dominant kernels: the two propagator passes
+    k_plane = ctx32.time_kernel(0, 50)
+    k_z = ctx32.time_kernel(1, 50)
+    k_den = ctx32.time_kernel(2, 10)
+    dom_name, (dom_ms, dom_bytes) = max((("prop_plane", k_plane), ("prop_z", k_z)),
+                                        key=lambda kv: kv[1][0])
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    ctx32.close()
+
+    ctx64, dt64, H64 = run_path(p, fields, scft_amd.PREC_FP64, args.warmup, args.steps)
+    it64 = args.steps / dt64
+    ctx64.close()
+
+    out = {
+        "metric": METRIC,
+        "value": it32,
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt32 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": WORKLOAD,
+            "grid": list(p.grid()),
+            "box_rg": [4.0, 4.0, 8.0],
+            "N": N,
+            "precision": "complex64 propagator/FFT, complex128 fields, FP64 Q/H",
+            "systems": 1,
+            "parallelism": "single GPU",
+        },
+        "fp64_plain_iter_s": it64,
+        "fp32_over_fp64": it32 / it64,
+        "dH_rel_fp32_vs_fp64": abs(H32 - H64) / abs(H64),
+        "H_fp64": [H64.real, H64.imag],
+        "iteration_roofline": {
+            "bytes_per_iter_fp32": b_iter(M, N, 8, 16),
+            "frac_fp32": b_iter(M, N, 8, 16) * it32 / (HBM_PEAK_GBS * 1e9),
+            "bytes_per_iter_fp64": b_iter(M, N, 16, 16),
+            "frac_fp64": b_iter(M, N, 16, 16) * it64 / (HBM_PEAK_GBS * 1e9),
+        },
+        "kernels_ms": {"prop_plane": k_plane[0], "prop_z": k_z[0], "density": k_den[0]},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, fields, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -156,6 +156,23 @@ int scft_calc_H(scft_ctx* ctx, double* H_re_im);
  * Captured into a HIP graph after the first call. */
 int scft_iterate(scft_ctx* ctx, int n_iter);
 
+/* Wait for all work queued on the context's device (hipDeviceSynchronize). */
+int scft_synchronize(scft_ctx* ctx);
+
+/* Measurement hook (no counterpart in the reference; used by the benchmark):
+ * times `reps` back-to-back launches of one kernel of the hot path with HIP
+ * events on the context's stream and returns the average duration and the
+ * algorithmic HBM bytes one launch moves (each array read or written once).
+ *   which = 0: propagator xy-plane pass (all chains, one contour step)
+ *   which = 1: propagator z-pencil pass with exp(-w) applied on store
+ *   which = 2: q.q-dagger density sum
+ *   which = 3: one full 1S iteration (update + densities), in ms, bytes per
+ *              the reference dataflow model
+ * The propagator passes recompute contour step 1 from step 0, so repeating
+ * them leaves the state unchanged. */
+int scft_time_kernel(scft_ctx* ctx, int which, int reps, double* avg_ms,
+                     double* bytes);
+
 /* Text of the last error on this thread. */
 const char* scft_last_error(void);
 

@@ -1,0 +1,267 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle on
+the same seeded inputs.
+
+Tolerances: the FP64 path must agree with the oracle to 1e-10 relative in H
+(the bar of the plain complex128 path) and to ~1e-12 in the stage outputs;
+the FP32-propagator path to 1e-6 relative in H.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle
+import scft_amd
+from scft_amd import PREC_FP32, PREC_FP64, Workload, seeded_fields
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+SMALL = dict(grid=(16, 16, 32), box_rg=(2.0, 2.0, 4.0), N=16)
+
+
+def pair(w: Workload, prec=PREC_FP64, seed=5, period=2.0):
+    p = w.params()
+    f = seeded_fields(p, seed, period_rg=period, amp=1.0, sigma=0.01)
+    ctx = scft_amd.Context(p, precision=prec)
+    ref = pyoracle.Oracle(p)
+    ctx.set_fields(*f)
+    ref.set_fields(*f)
+    return p, ctx, ref
+
+
+# ------------------------------------------------------------------ FFT seam
+@pytest.mark.parametrize("grid", [(8, 6, 10), (16, 16, 32), (12, 14, 1), (56, 64, 8), (72, 8, 6),
+                                  (96, 96, 4), (32, 32, 64)])
+def test_fft_seam_matches_oracle(grid):
+    say("fft", grid)
+    dim = 3 if grid[2] > 1 else 2
+    w = Workload(grid=grid if dim == 3 else grid[:2], box_rg=(2.0, 2.0, 2.0)[:dim], N=4, n_plates=0)
+    p = w.params()
+    ctx = scft_amd.Context(p)
+    ref = pyoracle.Oracle(p)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(p.M) + 1j * rng.standard_normal(p.M)
+    F = ctx.fft(x, -1)
+    assert relerr(F, ref.fft(x, -1)) < 1e-14
+    B = ctx.fft(x, +1)
+    assert relerr(B, ref.fft(x, +1)) < 1e-14
+    assert relerr(ctx.fft(F, +1), x) < 1e-14
+    ctx.close()
+
+
+# ------------------------------------------------------ set-up / bookkeeping
+@pytest.mark.parametrize("np_type,theta", [(scft_amd.NP_SQUARE, 0.0), (scft_amd.NP_ROD, 0.0),
+                                           (scft_amd.NP_SQUARE, 30.0)])
+def test_cavity_and_bookkeeping(np_type, theta):
+    say("cavity", np_type, theta)
+    w = Workload(grid=(32, 32, 64), box_rg=(4.0, 4.0, 8.0), N=50, np_type=np_type, theta_deg=theta)
+    p = w.params()
+    ctx = scft_amd.Context(p)
+    ref = pyoracle.Oracle(p)
+    assert relerr(ctx.get_cavity(), ref.array("rho_exp_nr")) < 1e-14
+    info = ctx.info()
+    for k in ("V", "V_1_exp_np", "np_frac", "phiD", "nD"):
+        assert abs(getattr(info, k) - ref.scalar(k)) <= 1e-13 * abs(ref.scalar(k)), k
+    ctx.close()
+
+
+# ------------------------------------------------- density / propagator (FP64)
+def test_poly_density_fp64():
+    say("poly_density fp64")
+    p, ctx, ref = pair(Workload(**SMALL))
+    ctx.calc_poly_density()
+    ref.calc_poly_density()
+    sa, sb = ctx.get_smeared()
+    assert relerr(sa, ref.array("smwa")) < 1e-13
+    assert relerr(sb, ref.array("smwb")) < 1e-13
+    Q = ctx.get_Q()
+    assert abs(Q - ref.Q()) < 1e-12 * abs(ref.Q())
+    rA, rB = ctx.get_density()
+    assert relerr(rA, ref.array("rhoda")) < 1e-11
+    assert relerr(rB, ref.array("rhodb")) < 1e-11
+    # segment count, code/simulate.cpp:61-72
+    s = ref.integ(rA + rB)
+    nDN = ref.scalar("nD") * p.N
+    assert abs(s - nDN) < 1e-10 * nDN
+    ctx.close()
+
+
+@pytest.mark.parametrize("f_A", [0.0, 0.3, 1.0])
+def test_poly_density_block_fractions(f_A):
+    # Na = 0 and Na = N make q-dagger the copy of q (code/diblock_discrete.cpp:81-83)
+    say("poly_density f_A", f_A)
+    p, ctx, ref = pair(Workload(grid=(8, 8, 16), box_rg=(2.0, 2.0, 4.0), N=10, f_A=f_A))
+    ctx.calc_poly_density()
+    ref.calc_poly_density()
+    assert abs(ctx.get_Q() - ref.Q()) < 1e-12 * abs(ref.Q())
+    rA, rB = ctx.get_density()
+    scale = max(np.max(np.abs(ref.array("rhoda"))), np.max(np.abs(ref.array("rhodb"))))
+    assert np.max(np.abs(rA - ref.array("rhoda"))) < 1e-11 * scale
+    assert np.max(np.abs(rB - ref.array("rhodb"))) < 1e-11 * scale
+    ctx.close()
+
+
+def test_update_1s_and_H_fp64():
+    say("update_1s fp64")
+    p, ctx, ref = pair(Workload(**SMALL))
+    ctx.calc_poly_density()
+    ref.calc_poly_density()
+    H0 = ctx.calc_H()[0]
+    assert abs(H0 - ref.calc_H()) < 1e-11 * abs(H0)
+    for _ in range(3):
+        ctx.update_1s()
+        ref.update_1s()
+    g = ctx.get_fields()
+    for a, name in zip(g, ("wpl", "wabp", "wabm")):
+        assert relerr(a, ref.array(name)) < 1e-11, name
+    H = ctx.calc_H()[0]
+    Hr = ref.calc_H()
+    assert abs(H - Hr) < 1e-11 * abs(Hr)
+    ctx.close()
+
+
+def test_negative_chi_branch_fp64():
+    say("negative chi")
+    w = Workload(grid=(8, 8, 16), box_rg=(2.0, 2.0, 4.0), N=10)
+    p = w.params()
+    p.chi_ab = -abs(p.chi_ab)
+    f = seeded_fields(p, 9, period_rg=2.0, amp=0.3, sigma=0.01)
+    ctx = scft_amd.Context(p)
+    ref = pyoracle.Oracle(p)
+    ctx.set_fields(*f)
+    ref.set_fields(*f)
+    ctx.calc_poly_density()
+    ref.calc_poly_density()
+    for _ in range(2):
+        ctx.update_1s()
+        ref.update_1s()
+    assert abs(ctx.calc_H()[0] - ref.calc_H()) < 1e-11 * abs(ref.calc_H())
+    ctx.close()
+
+
+def test_euler_update_fp64():
+    say("euler fp64")
+    p, ctx, ref = pair(Workload(**SMALL, update_scheme=0, dt=0.001))
+    ctx.calc_poly_density()
+    ref.calc_poly_density()
+    for _ in range(3):
+        ctx.update_euler()
+        ref.update_euler()
+    assert abs(ctx.calc_H()[0] - ref.calc_H()) < 1e-11 * abs(ref.calc_H())
+    ctx.close()
+
+
+def test_graph_iterate_equals_eager():
+    say("graph iterate")
+    w = Workload(**SMALL)
+    p = w.params()
+    f = seeded_fields(p, 5, period_rg=2.0)
+    a = scft_amd.Context(p)
+    b = scft_amd.Context(p)
+    for c in (a, b):
+        c.set_fields(*f)
+        c.calc_poly_density()
+    for _ in range(4):
+        a.update_1s()
+    b.iterate(4)
+    fa, fb = a.get_fields(), b.get_fields()
+    for x, y in zip(fa, fb):
+        assert np.array_equal(x, y)
+    # replays after eager work stay identical
+    a.update_1s()
+    b.iterate(1)
+    assert np.array_equal(a.get_fields()[2], b.get_fields()[2])
+    a.close()
+    b.close()
+
+
+def test_batched_uneven_systems_match_single():
+    say("batch uneven")
+    ws = [Workload(grid=(16, 16, 32), box_rg=(2.0, 2.0, 4.0), N=12, d=0.3),
+          Workload(grid=(24, 16, 32), box_rg=(3.0, 2.0, 4.0), N=12, d=0.25, theta_deg=10.0),
+          Workload(grid=(12, 14, 1)[:2], box_rg=(2.0, 2.5), N=12, n_plates=1)]
+    ps = [w.params() for w in ws]
+    fs = [seeded_fields(p, 20 + i, period_rg=2.0) for i, p in enumerate(ps)]
+    batch = scft_amd.Context(ps)
+    for i, f in enumerate(fs):
+        batch.set_fields(*f, sys=i)
+    batch.calc_poly_density()
+    batch.update_1s()
+    Hb = batch.calc_H()
+    for i, (p, f) in enumerate(zip(ps, fs)):
+        one = scft_amd.Context(p)
+        one.set_fields(*f)
+        one.calc_poly_density()
+        one.update_1s()
+        assert one.calc_H()[0] == Hb[i]
+        assert np.array_equal(one.get_fields()[0], batch.get_fields(i)[0])
+        one.close()
+    batch.close()
+
+
+# ------------------------------------------------------ trajectories (golden)
+def _golden(name):
+    here = os.path.join(os.path.dirname(__file__), "golden", name)
+    return json.load(open(here))
+
+
+def test_c1_trajectory_fp64_vs_oracle_fixture():
+    """Config 1 (32x32x64, N=50, 200 1S iterations) against H values the
+    oracle wrote (tests/golden/make_trajectory.py)."""
+    g = _golden("trajectory_32x32x64_N50.json")
+    w = Workload(**g["workload"])
+    p = w.params()
+    f = seeded_fields(p, g["seed"], period_rg=4.0, amp=1.0, sigma=0.01)
+    say("c1 trajectory fp64")
+    ctx = scft_amd.Context(p)
+    ctx.set_fields(*f)
+    ctx.calc_poly_density()
+    H0 = ctx.calc_H()[0]
+    assert abs(H0 - complex(*g["H0"])) < 1e-10 * abs(H0)
+    done = 0
+    for it, Hre, Him in g["H"]:
+        ctx.iterate(it + 1 - done)
+        done = it + 1
+        H = ctx.calc_H()[0]
+        ref = complex(Hre, Him)
+        say(" iter", it, H, ref, abs(H - ref) / abs(ref))
+        assert abs(H - ref) < 1e-10 * abs(ref)
+    rA, rB = ctx.get_density()
+    assert abs(np.sum(rA).real - g["sum_rhoA"]) < 1e-9 * abs(g["sum_rhoA"])
+    ctx.close()
+
+
+def test_c1_trajectory_fp32_within_1e6():
+    g = _golden("trajectory_32x32x64_N50.json")
+    w = Workload(**g["workload"])
+    p = w.params()
+    f = seeded_fields(p, g["seed"], period_rg=4.0, amp=1.0, sigma=0.01)
+    say("c1 trajectory fp32")
+    ctx = scft_amd.Context(p, precision=PREC_FP32)
+    ctx.set_fields(*f)
+    ctx.calc_poly_density()
+    done = 0
+    worst = 0.0
+    for it, Hre, Him in g["H"]:
+        ctx.iterate(it + 1 - done)
+        done = it + 1
+        H = ctx.calc_H()[0]
+        ref = complex(Hre, Him)
+        worst = max(worst, abs(H - ref) / abs(ref))
+    say(" fp32 worst |dH/H|", worst)
+    assert worst < 1e-6
+    ctx.close()
