@@ -300,8 +300,9 @@ def test_1s_uniform_mode_relaxation():
 def test_1s_fixed_point_is_stationary():
     # the 1S map leaves w unchanged iff F + (A - B) w = 0; iterate a small
     # system to its saddle point and check H and w stop changing.
-    w = _small(grid=(4, 4, 8), box_rg=(1.5, 1.5, 3.0), N=6, dt=0.02)
+    w = _small(grid=(4, 4, 8), box_rg=(1.5, 1.5, 3.0), N=6)
     p = w.params()
+    p.lam_imag = p.lam_real = 0.02   # reference units
     o = po.Oracle(p)
     wpl, wabp, wabm = seeded_fields(p, 11, period_rg=3.0, amp=0.3, sigma=0.01)
     o.set_fields(wpl, wabp, wabm)
