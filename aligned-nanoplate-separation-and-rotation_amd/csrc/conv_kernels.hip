@@ -1,0 +1,278 @@
+// Size-specialised passes: convolution IDFT[g(k) DFT[.]] (propagator,
+// smearing) and forward / backward DFTs (field stage) along x and y of one
+// xy plane (k_plane_op) and along z of a tile of z pencils (k_z_op), with
+// register-resident line transforms
+// (reg_fft.hpp).  LDS holds the tile between the coalesced global accesses
+// and the lane-group layout, and the twiddle / multiplier tables.
+//
+// Each size has several variants (lanes per line G, tile width); the host
+// autotuner times them on the real launch at context creation and keeps the
+// fastest.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "reg_fft.hpp"
+#include "scft_dev.hpp"
+
+namespace scft {
+
+constexpr int round64(int v) { return (v + 63) / 64 * 64; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+template <int NX, int NY, int GX, int GY>
+struct PlaneCfg {
+  static constexpr int T = round64(cmax(NY * GX, NX * GY));
+  static_assert(T <= 1024, "plane too large for one workgroup");
+  static_assert(NX % GX == 0 && NY % GY == 0, "lane split");
+};
+// LDS row stride in complex elements: 32 bytes of padding per row
+template <typename R, int NX>
+constexpr int plane_px() { return NX + (sizeof(R) == 4 ? 4 : 2); }
+
+template <typename R, int NX, int NY, int GX, int GY>
+constexpr size_t plane_lds_bytes() {
+  return sizeof(cpx<R>) * (NY * plane_px<R, NX>() + NX + NY) + sizeof(R) * (NX + NY);
+}
+
+// One xy plane per block: lines along x, then along y, each transformed by
+// line_op<MODE> (convolution IDFT[g DFT[.]], forward or backward DFT) in
+// lane groups; a coalesced epilogue applies the optional pointwise
+// multiplier (exp(-w) on the last pass of a propagator step) and the scale.
+template <typename R, int MODE, int NX, int NY, int GX, int GY>
+__global__ __launch_bounds__((PlaneCfg<NX, NY, GX, GY>::T)) void k_plane_op(
+    const PassDesc* __restrict__ descs, int step) {
+  constexpr int T = PlaneCfg<NX, NY, GX, GY>::T, PX = plane_px<R, NX>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* twx = lds + NY * PX;
+  cpx<R>* twy = twx + NX;
+  R* gx = reinterpret_cast<R*>(twy + NY);
+  R* gy = gx + NX;
+  const PassDesc& d = descs[blockIdx.y];
+  if (d.steps_active && step >= d.steps_active) return;
+  const int z = blockIdx.x;
+  if (z >= d.nz) return;
+  const int tid = threadIdx.x;
+  const long poff = (long)z * (NX * NY);
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step + d.in_soff) * d.in_step + poff;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step + d.out_soff) * d.out_step + poff;
+  if (tid < NX) {
+    twx[tid] = reinterpret_cast<const cpx<R>*>(d.tw0)[tid];
+    if (MODE == 0) gx[tid] = reinterpret_cast<const R*>(d.g0)[tid];
+  }
+  if (tid < NY) {
+    twy[tid] = reinterpret_cast<const cpx<R>*>(d.tw1)[tid];
+    if (MODE == 0) gy[tid] = reinterpret_cast<const R*>(d.g1)[tid];
+  }
+#pragma unroll 4
+  for (int i = tid; i < NX * NY; i += T) {
+    const int y = i / NX, x = i - y * NX;
+    lds[y * PX + x] = in[i];
+  }
+  __syncthreads();
+  {
+    const int grp = tid / GX, t = tid - grp * GX;
+    for (int row = grp; row < NY; row += T / GX) {
+      cpx<R>* line = lds + row * PX;
+      line_op<MODE, R, NX, GX>(t, twx, gx, [line](int i) { return line + i; });
+    }
+  }
+  __syncthreads();
+  if (NY > 1) {
+    const int grp = tid / GY, t = tid - grp * GY;
+    for (int col = grp; col < NX; col += T / GY) {
+      cpx<R>* line = lds + col;
+      line_op<MODE, R, NY, GY>(t, twy, gy, [line](int i) { return line + i * PX; });
+    }
+    __syncthreads();
+  }
+  const cpx<R>* post = nullptr;
+  R scale = R(1);
+  if (d.last) {
+    const void* pp = (step < d.post_switch) ? d.post_a : d.post_b;
+    if (pp) post = reinterpret_cast<const cpx<R>*>(pp) + poff;
+    scale = R(d.scale);
+  }
+#pragma unroll 4
+  for (int i = tid; i < NX * NY; i += T) {
+    const int y = i / NX, x = i - y * NX;
+    cpx<R> o = lds[y * PX + x];
+    if (post) o = cmul(o, post[i]);
+    if (scale != R(1)) o = cscale(o, scale);
+    out[i] = o;
+  }
+}
+
+template <int NZ, int G, int TL>
+struct ZCfg {
+  static constexpr int T = TL * G;
+  static constexpr int LP = TL + 1;  // LDS row stride (complex elements)
+  static_assert(NZ % G == 0 && T <= 1024, "z tile");
+};
+
+template <typename R, int NZ, int G, int TL>
+constexpr size_t z_lds_bytes() {
+  return sizeof(cpx<R>) * (NZ * ZCfg<NZ, G, TL>::LP + NZ) + sizeof(R) * NZ;
+}
+
+// A tile of TL consecutive z pencils (line l = x + nx y, stride nx ny):
+// coalesced load into LDS, line_op<MODE> along z in lane groups, then a
+// coalesced epilogue applying the optional multiplier and the scale.
+template <typename R, int MODE, int NZ, int G, int TL>
+__global__ __launch_bounds__((ZCfg<NZ, G, TL>::T)) void k_z_op(const PassDesc* __restrict__ descs,
+                                                               int step) {
+  using C = ZCfg<NZ, G, TL>;
+  constexpr int T = C::T, LP = C::LP;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* tw = lds + NZ * LP;
+  R* g = reinterpret_cast<R*>(tw + NZ);
+  const PassDesc& d = descs[blockIdx.y];
+  if (d.steps_active && step >= d.steps_active) return;
+  const int tid = threadIdx.x;
+  const int nlines = d.nx * d.ny;
+  const long S = nlines;
+  const int l0 = blockIdx.x * TL;
+  if (l0 >= nlines) return;
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step + d.in_soff) * d.in_step;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step + d.out_soff) * d.out_step;
+  for (int i = tid; i < NZ; i += T) {
+    tw[i] = reinterpret_cast<const cpx<R>*>(d.tw0)[i];
+    if (MODE == 0) g[i] = reinterpret_cast<const R*>(d.g0)[i];
+  }
+#pragma unroll 4
+  for (int i = tid; i < NZ * TL; i += T) {
+    const int zz = i / TL, l = i - zz * TL;
+    lds[zz * LP + l] = (l0 + l < nlines) ? in[(long)l0 + l + zz * S] : mk(R(0), R(0));
+  }
+  __syncthreads();
+  {
+    const int grp = tid / G, t = tid - grp * G;
+    cpx<R>* line = lds + grp;
+    line_op<MODE, R, NZ, G>(t, tw, g, [line](int i) { return line + i * LP; });
+  }
+  __syncthreads();
+  const cpx<R>* post = nullptr;
+  R scale = R(1);
+  if (d.last) {
+    const void* pp = (step < d.post_switch) ? d.post_a : d.post_b;
+    if (pp) post = reinterpret_cast<const cpx<R>*>(pp);
+    scale = R(d.scale);
+  }
+#pragma unroll 4
+  for (int i = tid; i < NZ * TL; i += T) {
+    const int zz = i / TL, l = i - zz * TL;
+    if (l0 + l >= nlines) continue;
+    const long gi = (long)l0 + l + zz * S;
+    cpx<R> o = lds[zz * LP + l];
+    if (post) o = cmul(o, post[gi]);
+    if (scale != R(1)) o = cscale(o, scale);
+    out[gi] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// variants: (NX, NY, GX, GY) for planes, (NZ, G, TL) for z tiles
+// ---------------------------------------------------------------------------
+#define SCFT_PLANE_VARIANTS(X)                                                             \
+  X(8, 8, 4, 4) X(16, 16, 4, 4) X(24, 16, 4, 4) X(32, 32, 4, 4) X(32, 32, 8, 8)            \
+  X(56, 64, 4, 4) X(56, 64, 8, 8) X(64, 64, 4, 4) X(64, 64, 8, 8) X(64, 64, 16, 16)        \
+  X(72, 64, 4, 4) X(72, 64, 8, 8) X(96, 96, 4, 4) X(96, 96, 8, 8)
+#define SCFT_Z_VARIANTS(X)                                                                 \
+  X(16, 4, 32) X(32, 4, 32) X(32, 8, 32) X(64, 8, 32) X(64, 8, 16) X(128, 8, 32)           \
+  X(128, 16, 32) X(128, 8, 16) X(128, 16, 16) X(192, 8, 32) X(192, 16, 32) X(192, 16, 16)
+
+template <typename R, int MODE, int NX, int NY, int GX, int GY>
+static hipError_t launch_plane_m(const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
+  using C = PlaneCfg<NX, NY, GX, GY>;
+  constexpr size_t lds = plane_lds_bytes<R, NX, NY, GX, GY>();
+  static bool attr = false;
+  if (!attr && lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)k_plane_op<R, MODE, NX, NY, GX, GY>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  attr = true;
+  hipLaunchKernelGGL((k_plane_op<R, MODE, NX, NY, GX, GY>), dim3(gridx, njobs), dim3(C::T), lds, st, d, step);
+  return hipGetLastError();
+}
+
+template <typename R, int NX, int NY, int GX, int GY>
+static hipError_t launch_plane_t(int mode, const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
+  if (mode == MODE_CONV) return launch_plane_m<R, MODE_CONV, NX, NY, GX, GY>(d, njobs, gridx, step, st);
+  if (mode == MODE_FWD) return launch_plane_m<R, MODE_FWD, NX, NY, GX, GY>(d, njobs, gridx, step, st);
+  return launch_plane_m<R, MODE_INV, NX, NY, GX, GY>(d, njobs, gridx, step, st);
+}
+
+template <typename R, int MODE, int NZ, int G, int TL>
+static hipError_t launch_z_m(const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
+  using C = ZCfg<NZ, G, TL>;
+  constexpr size_t lds = z_lds_bytes<R, NZ, G, TL>();
+  static bool attr = false;
+  if (!attr && lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)k_z_op<R, MODE, NZ, G, TL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  attr = true;
+  hipLaunchKernelGGL((k_z_op<R, MODE, NZ, G, TL>), dim3(gridx, njobs), dim3(C::T), lds, st, d, step);
+  return hipGetLastError();
+}
+
+template <typename R, int NZ, int G, int TL>
+static hipError_t launch_z_t(int mode, const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
+  if (mode == MODE_CONV) return launch_z_m<R, MODE_CONV, NZ, G, TL>(d, njobs, gridx, step, st);
+  if (mode == MODE_FWD) return launch_z_m<R, MODE_FWD, NZ, G, TL>(d, njobs, gridx, step, st);
+  return launch_z_m<R, MODE_INV, NZ, G, TL>(d, njobs, gridx, step, st);
+}
+
+int fast_plane_variants(int prec, int nx, int ny, int* ids, int max_ids) {
+  int n = 0, id = 0;
+#define X(a, b, ga, gb)                                                                             \
+  if (nx == a && ny == b && n < max_ids &&                                                          \
+      (prec == 0 ? plane_lds_bytes<double, a, b, ga, gb>() : plane_lds_bytes<float, a, b, ga, gb>()) <= \
+          160 * 1024)                                                                               \
+    ids[n++] = id;                                                                                  \
+  id++;
+  SCFT_PLANE_VARIANTS(X)
+#undef X
+  return n;
+}
+
+int fast_z_variants(int prec, int nz, int* ids, int* tiles, int max_ids) {
+  int n = 0, id = 0;
+#define X(a, g, tl)                                                                                 \
+  if (nz == a && n < max_ids &&                                                                     \
+      (prec == 0 ? z_lds_bytes<double, a, g, tl>() : z_lds_bytes<float, a, g, tl>()) <= 160 * 1024) { \
+    tiles[n] = tl;                                                                                  \
+    ids[n++] = id;                                                                                  \
+  }                                                                                                 \
+  id++;
+  SCFT_Z_VARIANTS(X)
+#undef X
+  return n;
+}
+
+hipError_t launch_fast_plane(int prec, int mode, int variant, const PassDesc* d, int njobs, int gridx,
+                             int step, hipStream_t st) {
+  int id = 0;
+#define X(a, b, ga, gb)                                                                        \
+  if (variant == id)                                                                           \
+    return prec == 0 ? launch_plane_t<double, a, b, ga, gb>(mode, d, njobs, gridx, step, st)   \
+                     : launch_plane_t<float, a, b, ga, gb>(mode, d, njobs, gridx, step, st);   \
+  id++;
+  SCFT_PLANE_VARIANTS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_fast_z(int prec, int mode, int variant, const PassDesc* d, int njobs, int gridx, int step,
+                         hipStream_t st) {
+  int id = 0;
+#define X(a, g, tl)                                                                          \
+  if (variant == id)                                                                         \
+    return prec == 0 ? launch_z_t<double, a, g, tl>(mode, d, njobs, gridx, step, st)         \
+                     : launch_z_t<float, a, g, tl>(mode, d, njobs, gridx, step, st);         \
+  id++;
+  SCFT_Z_VARIANTS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+}  // namespace scft

@@ -12,6 +12,14 @@ struct SysDev;
 // plane: k_plane (x and y of one xy plane per block) or k_line (one axis).
 hipError_t launch_pass(int prec, int mode, bool plane, const PassDesc* d, int njobs, int gridx,
                        int tb, size_t lds, int step, hipStream_t st);
+// size-specialised convolution passes (conv_kernels.hip): variant ids of a
+// size, then launches by id
+int fast_plane_variants(int prec, int nx, int ny, int* ids, int max_ids);
+int fast_z_variants(int prec, int nz, int* ids, int* tiles, int max_ids);
+hipError_t launch_fast_plane(int prec, int mode, int variant, const PassDesc* d, int njobs, int gridx,
+                             int step, hipStream_t st);
+hipError_t launch_fast_z(int prec, int mode, int variant, const PassDesc* d, int njobs, int gridx, int step,
+                         hipStream_t st);
 hipError_t launch_assemble(const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_exp_prop(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_density(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);

@@ -107,10 +107,17 @@ struct DevBuf {
 // ---------------------------------------------------------------------------
 struct Launch {
   bool plane;
+  int fast;  // 0 generic, 1 size-specialised plane conv, 2 size-specialised z conv
+  int n0, n1;
   int mode, prec;
   PassDesc* d;
   int njobs, gridx, tb;
   size_t lds;
+  // size-specialised launches: candidate variants, z tile widths, choice
+  std::vector<int> variants, tiles;
+  std::vector<float> tuned_ms;
+  int choice = 0;
+  int max_lines = 0;
 };
 
 struct Program {
@@ -210,12 +217,26 @@ int build_program(scft_ctx* c, int mode, const std::vector<JobSpec>& jobs, Progr
     }
     if (s.nz > 1) q.push_back(ST_Z);
   }
-  // group by (stage index, kind, prec)
-  std::map<std::tuple<int, int, int>, std::vector<Item>> groups;
-  for (size_t j = 0; j < jobs.size(); j++)
-    for (size_t k = 0; k < seq[j].size(); k++)
-      groups[{(int)k, seq[j][k], jobs[j].prec}].push_back(
-          {(int)j, (int)k, seq[j][k], k + 1 == seq[j].size()});
+  // group by (stage index, kind, prec, specialised size)
+  std::map<std::tuple<int, int, int, int, int, int>, std::vector<Item>> groups;
+  for (size_t j = 0; j < jobs.size(); j++) {
+    const SysHost& s = c->sys[jobs[j].sys];
+    for (size_t k = 0; k < seq[j].size(); k++) {
+      const int kind = seq[j][k];
+      int fast = 0, n0 = 0, n1 = 0;
+      int ids[16], tls[16];
+      if (kind == ST_PLANE && scft::fast_plane_variants(jobs[j].prec, s.nx, s.ny, ids, 16) > 0) {
+        fast = 1;
+        n0 = s.nx;
+        n1 = s.ny;
+      } else if (kind == ST_Z && scft::fast_z_variants(jobs[j].prec, s.nz, ids, tls, 16) > 0) {
+        fast = 2;
+        n0 = s.nz;
+      }
+      groups[{(int)k, kind, jobs[j].prec, fast, n0, n1}].push_back(
+          {(int)j, (int)k, kind, k + 1 == seq[j].size()});
+    }
+  }
   std::vector<PassDesc> all;
   std::vector<Launch> launches;
   std::vector<size_t> starts;
@@ -225,6 +246,18 @@ int build_program(scft_ctx* c, int mode, const std::vector<JobSpec>& jobs, Progr
     const size_t esz = prec == 0 ? 16 : 8;
     Launch L{};
     L.plane = kind == ST_PLANE;
+    L.fast = std::get<3>(kv.first);
+    L.n0 = std::get<4>(kv.first);
+    L.n1 = std::get<5>(kv.first);
+    if (L.fast) {
+      int ids[16], tls[16];
+      const int nv = L.fast == 1 ? scft::fast_plane_variants(prec, L.n0, L.n1, ids, 16)
+                                 : scft::fast_z_variants(prec, L.n0, ids, tls, 16);
+      for (int v = 0; v < nv; v++) {
+        L.variants.push_back(ids[v]);
+        L.tiles.push_back(L.fast == 2 ? tls[v] : 0);
+      }
+    }
     L.mode = mode;
     L.prec = prec;
     L.njobs = (int)kv.second.size();
@@ -278,6 +311,11 @@ int build_program(scft_ctx* c, int mode, const std::vector<JobSpec>& jobs, Progr
         d.tw0 = tw_ptr(c, n, prec);
         d.g0 = gsel(axis);
         const int nlines = s.M / n;
+        if (L.fast == 2) {
+          L.max_lines = std::max(L.max_lines, nlines);
+          all.push_back(d);
+          continue;
+        }
         const int budget = prec == 0 ? 2048 : 4096;
         int tl = std::max(1, budget / n);
         if (tl >= 8) tl &= ~7;
@@ -315,10 +353,53 @@ int build_program(scft_ctx* c, int mode, const std::vector<JobSpec>& jobs, Progr
   return SCFT_OK;
 }
 
-int run_program(scft_ctx* c, const Program& p, int step) {
-  for (const Launch& L : p.launches) {
-    HIPCK(scft::launch_pass(L.prec, L.mode, L.plane, L.d, L.njobs, L.gridx, L.tb, L.lds, step, c->st));
+hipError_t launch_one(scft_ctx* c, const Launch& L, int step) {
+  if (L.fast == 1)
+    return scft::launch_fast_plane(L.prec, L.mode, L.variants[L.choice], L.d, L.njobs, L.gridx, step, c->st);
+  if (L.fast == 2) {
+    const int tl = L.tiles[L.choice];
+    return scft::launch_fast_z(L.prec, L.mode, L.variants[L.choice], L.d, L.njobs, (L.max_lines + tl - 1) / tl,
+                               step, c->st);
   }
+  return scft::launch_pass(L.prec, L.mode, L.plane, L.d, L.njobs, L.gridx, L.tb, L.lds, step, c->st);
+}
+
+// Autotuner: time every variant of each size-specialised launch of a
+// program on the context's own buffers (step 1 of the propagator recomputes
+// q_1 from q_0; the smearing works in place on scratch fields) and keep the
+// fastest.
+int autotune(scft_ctx* c, Program& p, int step) {
+  hipEvent_t e0, e1;
+  HIPCK(hipEventCreate(&e0));
+  HIPCK(hipEventCreate(&e1));
+  for (Launch& L : p.launches) {
+    if (L.variants.size() < 2) continue;
+    float best = 1e30f;
+    int best_i = 0;
+    for (size_t v = 0; v < L.variants.size(); v++) {
+      L.choice = (int)v;
+      for (int r = 0; r < 2; r++) HIPCK(launch_one(c, L, step));
+      HIPCK(hipEventRecord(e0, c->st));
+      for (int r = 0; r < 8; r++) HIPCK(launch_one(c, L, step));
+      HIPCK(hipEventRecord(e1, c->st));
+      HIPCK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCK(hipEventElapsedTime(&ms, e0, e1));
+      L.tuned_ms.push_back(ms / 8.f);
+      if (ms < best) {
+        best = ms;
+        best_i = (int)v;
+      }
+    }
+    L.choice = best_i;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return SCFT_OK;
+}
+
+int run_program(scft_ctx* c, const Program& p, int step) {
+  for (const Launch& L : p.launches) HIPCK(launch_one(c, L, step));
   return SCFT_OK;
 }
 
@@ -843,6 +924,49 @@ int scft_create(const scft_params* params, int n_systems, int device, int precis
       rc = fail(SCFT_EDEVICE, "copy failed");
   }
   if (rc == SCFT_OK && hipStreamSynchronize(c->st) != hipSuccess) rc = fail(SCFT_EDEVICE, "set-up failed on the device");
+  if (rc == SCFT_OK && !getenv("SCFT_NO_TUNE")) {
+    // tuning runs on zeroed propagators and scratch fields (recomputed later)
+    for (auto& s : c->sys) {
+      if (rc == SCFT_OK && hipMemsetAsync(s.dev.q, 0, (size_t)2 * s.M * c->esz, c->st) != hipSuccess)
+        rc = fail(SCFT_EDEVICE, "memset failed");
+      if (rc == SCFT_OK && s.dev.qd != s.dev.q &&
+          hipMemsetAsync(s.dev.qd, 0, (size_t)2 * s.M * c->esz, c->st) != hipSuccess)
+        rc = fail(SCFT_EDEVICE, "memset failed");
+      if (rc == SCFT_OK && hipMemsetAsync(s.dev.wa, 0, (size_t)s.M * 16, c->st) != hipSuccess)
+        rc = fail(SCFT_EDEVICE, "memset failed");
+      if (rc == SCFT_OK && hipMemsetAsync(s.dev.wb, 0, (size_t)s.M * 16, c->st) != hipSuccess)
+        rc = fail(SCFT_EDEVICE, "memset failed");
+    }
+    if (rc == SCFT_OK) rc = autotune(c, c->prog_prop, 1);
+    if (rc == SCFT_OK) rc = autotune(c, c->prog_smear, 0);
+    // the field programs work in place on w+, wAB+-, rhoA, rhoB: keep a copy
+    std::vector<void*> saved;
+    for (auto& s : c->sys) {
+      void* p = nullptr;
+      if (rc == SCFT_OK && hipMalloc(&p, (size_t)5 * s.M * 16) != hipSuccess)
+        rc = fail(SCFT_ENOMEM, "no memory for the tuning copy");
+      saved.push_back(p);
+      void* src[5] = {s.dev.wpl, s.dev.wabp, s.dev.wabm, s.dev.rhoA, s.dev.rhoB};
+      for (int a = 0; a < 5 && rc == SCFT_OK; a++)
+        if (hipMemcpyAsync((char*)p + (size_t)a * s.M * 16, src[a], (size_t)s.M * 16, hipMemcpyDeviceToDevice,
+                           c->st) != hipSuccess)
+          rc = fail(SCFT_EDEVICE, "copy failed");
+    }
+    if (rc == SCFT_OK) rc = autotune(c, c->prog_fwd, 0);
+    if (rc == SCFT_OK) rc = autotune(c, c->prog_inv, 0);
+    for (size_t i = 0; i < c->sys.size(); i++) {
+      auto& s = c->sys[i];
+      void* dst[5] = {s.dev.wpl, s.dev.wabp, s.dev.wabm, s.dev.rhoA, s.dev.rhoB};
+      for (int a = 0; a < 5 && rc == SCFT_OK && saved[i]; a++)
+        if (hipMemcpyAsync(dst[a], (char*)saved[i] + (size_t)a * s.M * 16, (size_t)s.M * 16,
+                           hipMemcpyDeviceToDevice, c->st) != hipSuccess)
+          rc = fail(SCFT_EDEVICE, "copy failed");
+    }
+    (void)hipStreamSynchronize(c->st);
+    for (void* p : saved)
+      if (p) (void)hipFree(p);
+    if (rc == SCFT_OK && hipStreamSynchronize(c->st) != hipSuccess) rc = fail(SCFT_EDEVICE, "autotune failed");
+  }
   if (rc != SCFT_OK) {
     std::string msg = g_err;
     scft_destroy(c);
@@ -1013,13 +1137,12 @@ int scft_time_kernel(scft_ctx* c, int which, int reps, double* avg_ms, double* b
   if (reps < 1 || !avg_ms || !bytes) return fail(SCFT_EINVAL, "bad arguments");
   const int ns = (int)c->sys.size();
   double b = 0.0;
-  auto launch_one = [&]() -> int {
+  auto run_once = [&]() -> int {
     if (which == 0 || which == 1) {
       const Program& p = c->prog_prop;
       const size_t k = which == 0 ? 0 : 1;
       if (k >= p.launches.size()) return fail(SCFT_EINVAL, "this grid has no separate z pass");
-      const Launch& L = p.launches[k];
-      HIPCK(scft::launch_pass(L.prec, L.mode, L.plane, L.d, L.njobs, L.gridx, L.tb, L.lds, 1, c->st));
+      HIPCK(launch_one(c, p.launches[k], 1));
       return SCFT_OK;
     }
     if (which == 2) {
@@ -1041,12 +1164,12 @@ int scft_time_kernel(scft_ctx* c, int which, int reps, double* avg_ms, double* b
       b += M * (10.0 * (N - 1) * pe + 2 * pe + 2 * N * pe + 2 * 16.0 + 24 * 16.0 + 12 * 16.0);
     }
   }
-  if ((rc = launch_one())) return rc;  // warm
+  if ((rc = run_once())) return rc;  // warm
   hipEvent_t e0, e1;
   HIPCK(hipEventCreate(&e0));
   HIPCK(hipEventCreate(&e1));
   HIPCK(hipEventRecord(e0, c->st));
-  for (int r = 0; r < reps && rc == SCFT_OK; r++) rc = launch_one();
+  for (int r = 0; r < reps && rc == SCFT_OK; r++) rc = run_once();
   HIPCK(hipEventRecord(e1, c->st));
   HIPCK(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -1056,6 +1179,44 @@ int scft_time_kernel(scft_ctx* c, int which, int reps, double* avg_ms, double* b
   if (rc) return rc;
   *avg_ms = ms / reps;
   *bytes = b;
+  return SCFT_OK;
+}
+
+int scft_describe(scft_ctx* c, char* buf, int len) {
+  int rc = check_ctx(c, 0);
+  if (rc) return rc;
+  if (!buf || len < 1) return fail(SCFT_EINVAL, "bad buffer");
+  std::string o;
+  auto prog = [&](const char* name, const Program& p) {
+    o += name;
+    o += ":";
+    for (const Launch& L : p.launches) {
+      char tmp[256];
+      if (L.fast == 1)
+        snprintf(tmp, sizeof tmp, " plane_conv[%dx%d prec%d jobs%d variant %d of %zu]", L.n0, L.n1, L.prec,
+                 L.njobs, L.variants[L.choice], L.variants.size());
+      else if (L.fast == 2)
+        snprintf(tmp, sizeof tmp, " z_conv[%d tile%d prec%d jobs%d variant %d of %zu]", L.n0, L.tiles[L.choice],
+                 L.prec, L.njobs, L.variants[L.choice], L.variants.size());
+      else
+        snprintf(tmp, sizeof tmp, " %s[mode%d prec%d jobs%d tb%d lds%zu]", L.plane ? "plane" : "line", L.mode,
+                 L.prec, L.njobs, L.tb, L.lds);
+      o += tmp;
+      if (!L.tuned_ms.empty()) {
+        o += " tuned_us=";
+        for (float t : L.tuned_ms) {
+          snprintf(tmp, sizeof tmp, "%.1f,", 1000.0 * t);
+          o += tmp;
+        }
+      }
+    }
+    o += "\n";
+  };
+  prog("smear", c->prog_smear);
+  prog("prop", c->prog_prop);
+  prog("field_fwd", c->prog_fwd);
+  prog("field_inv", c->prog_inv);
+  snprintf(buf, len, "%s", o.c_str());
   return SCFT_OK;
 }
 

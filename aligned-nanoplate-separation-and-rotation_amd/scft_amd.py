@@ -270,6 +270,7 @@ def load_library(path: str | None = None):
         "scft_calc_H": (ctypes.c_int, [vp, dp]),
         "scft_iterate": (ctypes.c_int, [vp, ctypes.c_int]),
         "scft_synchronize": (ctypes.c_int, [vp]),
+        "scft_describe": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
         "scft_time_kernel": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, dp, dp]),
         "scft_last_error": (ctypes.c_char_p, []),
     }
@@ -392,6 +393,11 @@ class Context:
 
     def iterate(self, n: int):
         self._check(self.lib.scft_iterate(self.h, n), "scft_iterate")
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(8192)
+        self._check(self.lib.scft_describe(self.h, buf, len(buf)), "scft_describe")
+        return buf.value.decode()
 
     def synchronize(self):
         self._check(self.lib.scft_synchronize(self.h), "scft_synchronize")

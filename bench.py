@@ -114,6 +114,7 @@ def main():
     dom_name, (dom_ms, dom_bytes) = max((("prop_plane", k_plane), ("prop_z", k_z)),
                                         key=lambda kv: kv[1][0])
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    plan32 = ctx32.describe()
     ctx32.close()
 
     ctx64, dt64, H64 = run_path(p, fields, scft_amd.PREC_FP64, args.warmup, args.steps)
@@ -166,6 +167,7 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, fields, args.cpu_budget)
+    print(plan32, file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -173,6 +173,11 @@ int scft_synchronize(scft_ctx* ctx);
 int scft_time_kernel(scft_ctx* ctx, int which, int reps, double* avg_ms,
                      double* bytes);
 
+/* Text description of the launch plan (kernels, sizes, the variant the
+ * autotuner chose for each size-specialised pass); writes at most len bytes
+ * including the terminating zero.  No counterpart in the reference. */
+int scft_describe(scft_ctx* ctx, char* buf, int len);
+
 /* Text of the last error on this thread. */
 const char* scft_last_error(void);
 

@@ -79,6 +79,26 @@ def test_cavity_and_bookkeeping(np_type, theta):
     ctx.close()
 
 
+@pytest.mark.parametrize("ic", [-2, -1])
+def test_init_fields_survive_setup(ic):
+    # init_fields (code/initialize.cpp:419-565) with the tanh-cosine (-2) and
+    # the ran2 uniform (-1) starts; set-up work (autotuning included) must
+    # leave them intact
+    say("init fields", ic)
+    w = Workload(grid=(16, 16, 32), box_rg=(2.0, 2.0, 4.0), N=16)
+    p = w.params()
+    p.ic_flag[0], p.ic_pre[0], p.ic_dir[0], p.ic_period[0] = ic, 0.8, 2.0, 2.0
+    p.ic_flag[1], p.ic_pre[1], p.ic_dir[1], p.ic_period[1] = -4, 1.0, 0.0, 1.0
+    ctx = scft_amd.Context(p)
+    ref = pyoracle.Oracle(p)
+    g = ctx.get_fields()
+    for a, name in zip(g, ("wpl", "wabp", "wabm")):
+        r = ref.array(name)
+        assert np.max(np.abs(a - r)) <= 1e-15 * max(1.0, np.max(np.abs(r))), name
+    assert np.max(np.abs(g[2])) > 0.1
+    ctx.close()
+
+
 # ------------------------------------------------- density / propagator (FP64)
 def test_poly_density_fp64():
     say("poly_density fp64")
@@ -207,8 +227,9 @@ def test_batched_uneven_systems_match_single():
         one.set_fields(*f)
         one.calc_poly_density()
         one.update_1s()
-        assert one.calc_H()[0] == Hb[i]
-        assert np.array_equal(one.get_fields()[0], batch.get_fields(i)[0])
+        # the autotuner may pick another lane split in a batch: rounding only
+        assert abs(one.calc_H()[0] - Hb[i]) <= 1e-13 * abs(Hb[i])
+        assert relerr(one.get_fields()[0], batch.get_fields(i)[0]) < 1e-13
         one.close()
     batch.close()
 
