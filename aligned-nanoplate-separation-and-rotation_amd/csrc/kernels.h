@@ -20,6 +20,7 @@ hipError_t launch_fast_plane(int prec, int mode, int variant, const PassDesc* d,
                              int step, hipStream_t st);
 hipError_t launch_fast_z(int prec, int mode, int variant, const PassDesc* d, int njobs, int gridx, int step,
                          hipStream_t st);
+hipError_t launch_copy16(const void* src, void* dst, long n16, hipStream_t st);
 hipError_t launch_assemble(const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_exp_prop(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_density(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);

@@ -1149,10 +1149,19 @@ int scft_time_kernel(scft_ctx* c, int which, int reps, double* avg_ms, double* b
       HIPCK(scft::launch_density(c->prec, c->d_sys, ns, c->maxM, c->st));
       return SCFT_OK;
     }
+    if (which == 4) {
+      const SysHost& s0 = c->sys[0];
+      const long n16 = (long)(2 * (size_t)s0.M * c->esz / 16);
+      HIPCK(scft::launch_copy16(s0.dev.q, (char*)s0.dev.q + (size_t)2 * s0.M * c->esz, n16, c->st));
+      return SCFT_OK;
+    }
     return stage_update(c, c->sys[0].p.update_scheme);
   };
+  if (which == 4 && c->sys[0].N < 4) return fail(SCFT_EINVAL, "the copy needs N >= 4");
   // algorithmic bytes of one launch
+  if (which == 4) b = 4.0 * c->sys[0].M * (double)c->esz;
   for (auto& s : c->sys) {
+    if (which == 4) break;
     const double M = s.M, pe = (double)c->esz;
     const int chains = (s.dev.qd != s.dev.q) ? 2 : 1;
     const bool two_pass = s.nz > 1;

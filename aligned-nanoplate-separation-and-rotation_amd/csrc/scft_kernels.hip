@@ -494,6 +494,17 @@ hipError_t launch_pass(int prec, int mode, bool plane, const PassDesc* d, int nj
 
 static inline dim3 pgrid(int maxM, int nsys) { return dim3((maxM + 255) / 256, nsys); }
 
+// Plain 16-byte copy: the calibration kernel of known traffic for the
+// hardware byte counters (read n16 * 16 bytes, write n16 * 16 bytes).
+__global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ a, float4* __restrict__ b, long n16) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n16; i += (long)gridDim.x * 256) b[i] = a[i];
+}
+
+hipError_t launch_copy16(const void* src, void* dst, long n16, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy16, dim3(1024), dim3(256), 0, st, (const float4*)src, (float4*)dst, n16);
+  return hipGetLastError();
+}
+
 hipError_t launch_assemble(const SysDev* s, int nsys, int maxM, hipStream_t st) {
   hipLaunchKernelGGL(k_assemble, pgrid(maxM, nsys), dim3(256), 0, st, s);
   return hipGetLastError();

@@ -208,6 +208,36 @@ def config(name: str, **over) -> Workload:
     return w
 
 
+def sweep(name: str, **over) -> list:
+    """Workloads of the batched sweeps (one context holds all of them).
+
+    "separation-64": 16 separations d = 0.14, 0.16, ..., 0.44 on the
+      production 64 x 64 x 128 grid (N = 100), the headline batch;
+    "config3": the 48-system sweep, 33 separations d = 0.14..0.46 (step 0.01)
+      at theta = 0 and 15 angles {0..10, 20, 30, 45, 90} deg at d = 0.30,
+      x grid cycled 56 / 64 / 72 with Lx = 3.5 / 4.0 / 4.5 Rg, 64 x 128 in y, z.
+    """
+    out = []
+    if name == "separation-64":
+        for i in range(16):
+            w = config("production-64x64x128-N100", d=round(0.14 + 0.02 * i, 4), seed=1000 + i)
+            out.append(w)
+    elif name == "config3":
+        geoms = [(round(0.14 + 0.01 * i, 4), 0.0) for i in range(33)]
+        geoms += [(0.30, float(a)) for a in list(range(11)) + [20, 30, 45, 90]]
+        xs = [(56, 3.5), (64, 4.0), (72, 4.5)]
+        for i, (d, th) in enumerate(geoms):
+            nx, lx = xs[i % 3]
+            out.append(Workload(grid=(nx, 64, 128), box_rg=(lx, 4.0, 8.0), N=100, d=d, theta_deg=th,
+                                itermax=1999, seed=1000 + i))
+    else:
+        raise KeyError(name)
+    for w in out:
+        for k, v in over.items():
+            setattr(w, k, v)
+    return out
+
+
 def seeded_fields(p: ScftParams, seed: int, period_rg: float = 4.0,
                   amp: float = 1.0, sigma: float = 0.01):
     """The lamellar start: wAB- = amp cos(2 pi z / period) + N(0, sigma^2);

@@ -2,6 +2,12 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
+Headline (`value`): the FP32-propagator path on a batch of 16 production
+systems (separations d = 0.14 ... 0.44) held in one context, in
+system-iterations/s; each step advances all 16 systems by one iteration.
+Beside it: the single production system with the FP32 propagator and with
+the plain FP64 path, their ratio, and |dH/H| between the two.
+
 A step is one SCFT iteration of the production system (64 x 64 x 128 grid,
 box 4 x 4 x 8 Rg, AB diblock N = 100, f_A = 0.5, chi N = 20, C = 50,
 kappa N = 50, two square plates at d = 0.30): the 1S semi-implicit field
@@ -41,6 +47,17 @@ def b_iter(M, N, b, bf):
     """Algorithmic bytes of one SCFT iteration (reference dataflow: q and
     q-dagger stored, each 3D FFT one read + one write)."""
     return M * (10 * (N - 1) * b + 2 * b + 2 * N * b + 2 * bf + 24 * bf + 12 * bf)
+
+
+def traffic_for(kernel, systems=1):
+    """HBM bytes per launch of a propagator pass from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated on a plain copy),
+    per system times the systems of one launch."""
+    path = os.path.join(REPO, "profiles", "traffic_64x64x128_N100.json")
+    try:
+        return systems * json.load(open(path))["kernels"][kernel]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def run_path(p, fields, prec, warmup, steps):
@@ -87,6 +104,21 @@ def cpu_baseline(p, fields, budget_s=20.0):
     }
 
 
+def run_batch(ws, prec, warmup, steps):
+    ps = [w.params() for w in ws]
+    ctx = scft_amd.Context(ps, device=0, precision=prec)
+    for i, (w, p) in enumerate(zip(ws, ps)):
+        ctx.set_fields(*scft_amd.seeded_fields(p, w.seed, period_rg=4.0, amp=1.0, sigma=0.01), sys=i)
+    ctx.calc_poly_density()
+    ctx.iterate(max(warmup, 1))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.iterate(steps)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    return ctx, dt, ps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,74 +132,88 @@ def main():
               "(independent systems shard across GPUs without exchange; no multi-GPU run here).")
         sys.exit(2)
 
+    # headline: FP32 propagator, a batch of 16 production systems in one context
+    batch = scft_amd.sweep("separation-64")
+    ctxb, dtb, psb = run_batch(batch, scft_amd.PREC_FP32, args.warmup, args.steps)
+    nb = len(batch)
+    batched = nb * args.steps / dtb
+    k_plane = ctxb.time_kernel(0, 20)
+    k_z = ctxb.time_kernel(1, 20)
+    k_den = ctxb.time_kernel(2, 5)
+    dom_name, (dom_ms, dom_bytes) = max((("prop_plane", k_plane), ("prop_z", k_z)),
+                                        key=lambda kv: kv[1][0])
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    planb = ctxb.describe()
+    ctxb.close()
+
+    # the single production system, FP32 propagator and plain FP64
     w = scft_amd.config(WORKLOAD)
     p = w.params()
     fields = scft_amd.seeded_fields(p, w.seed, period_rg=4.0, amp=1.0, sigma=0.01)
     M, N = p.M, p.N
-
     ctx32, dt32, H32 = run_path(p, fields, scft_amd.PREC_FP32, args.warmup, args.steps)
     it32 = args.steps / dt32
-    # dominant kernels: the two propagator passes
-    k_plane = ctx32.time_kernel(0, 50)
-    k_z = ctx32.time_kernel(1, 50)
-    k_den = ctx32.time_kernel(2, 10)
-    dom_name, (dom_ms, dom_bytes) = max((("prop_plane", k_plane), ("prop_z", k_z)),
-                                        key=lambda kv: kv[1][0])
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    plan32 = ctx32.describe()
+    s_plane = ctx32.time_kernel(0, 50)
+    s_z = ctx32.time_kernel(1, 50)
+    ctx32.time_kernel(4, 20)   # counter-calibration copy for the rocprof runs
     ctx32.close()
-
     ctx64, dt64, H64 = run_path(p, fields, scft_amd.PREC_FP64, args.warmup, args.steps)
     it64 = args.steps / dt64
     ctx64.close()
 
     out = {
         "metric": METRIC,
-        "value": it32,
+        "value": batched,
         "unit": "iter/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * dt32 / args.steps,
+        "ms_per_step": 1e3 * dtb / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": WORKLOAD,
+            "workload": "batched-16x-production-64x64x128-N100",
+            "systems": nb,
             "grid": list(p.grid()),
             "box_rg": [4.0, 4.0, 8.0],
             "N": N,
+            "separations": [x.d for x in batch],
             "precision": "complex64 propagator/FFT, complex128 fields, FP64 Q/H",
-            "systems": 1,
-            "parallelism": "single GPU",
+            "parallelism": "single GPU, systems batched per launch",
         },
+        "value_definition": "system-iterations/s of the FP32 batch (16 systems x 1 iteration per step)",
+        "single_system_fp32_iter_s": it32,
         "fp64_plain_iter_s": it64,
-        "fp32_over_fp64": it32 / it64,
+        "fp32_batched_over_fp64_plain": batched / it64,
+        "fp32_single_over_fp64_plain": it32 / it64,
         "dH_rel_fp32_vs_fp64": abs(H32 - H64) / abs(H64),
         "H_fp64": [H64.real, H64.imag],
         "iteration_roofline": {
             "bytes_per_iter_fp32": b_iter(M, N, 8, 16),
-            "frac_fp32": b_iter(M, N, 8, 16) * it32 / (HBM_PEAK_GBS * 1e9),
+            "frac_fp32_batched": b_iter(M, N, 8, 16) * batched / (HBM_PEAK_GBS * 1e9),
+            "frac_fp32_single": b_iter(M, N, 8, 16) * it32 / (HBM_PEAK_GBS * 1e9),
             "bytes_per_iter_fp64": b_iter(M, N, 16, 16),
-            "frac_fp64": b_iter(M, N, 16, 16) * it64 / (HBM_PEAK_GBS * 1e9),
+            "frac_fp64_single": b_iter(M, N, 16, 16) * it64 / (HBM_PEAK_GBS * 1e9),
         },
-        "kernels_ms": {"prop_plane": k_plane[0], "prop_z": k_z[0], "density": k_den[0]},
+        "kernels_ms": {"batch_prop_plane": k_plane[0], "batch_prop_z": k_z[0], "batch_density": k_den[0],
+                       "single_prop_plane": s_plane[0], "single_prop_z": s_z[0]},
         "roofline": {
             "bound": "hbm",
-            "kernel": dom_name,
+            "kernel": dom_name + " (16-system batch, one launch per contour step)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_for(dom_name, nb),
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, fields, args.cpu_budget)
-    print(plan32, file=sys.stderr, flush=True)
+    print(planb, file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -168,6 +168,8 @@ int scft_synchronize(scft_ctx* ctx);
  *   which = 2: q.q-dagger density sum
  *   which = 3: one full 1S iteration (update + densities), in ms, bytes per
  *              the reference dataflow model
+ *   which = 4: a plain copy of 2*M propagator values of system 0 (from its
+ *              contour slices 0-1 to 2-3): calibration of byte counters
  * The propagator passes recompute contour step 1 from step 0, so repeating
  * them leaves the state unchanged. */
 int scft_time_kernel(scft_ctx* ctx, int which, int reps, double* avg_ms,

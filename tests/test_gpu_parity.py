@@ -234,6 +234,33 @@ def test_batched_uneven_systems_match_single():
     batch.close()
 
 
+def test_config3_sweep_one_launch():
+    # the 48-system sweep of uneven boxes (x grid 56 / 64 / 72) in one
+    # context, FP32 propagator: every system advances, and the batch agrees
+    # with single-system runs to FP32 rounding
+    say("config3 sweep")
+    ws = scft_amd.sweep("config3")
+    ps = [w.params() for w in ws]
+    batch = scft_amd.Context(ps, precision=PREC_FP32)
+    fs = [seeded_fields(p, w.seed, period_rg=4.0) for w, p in zip(ws, ps)]
+    for i, f in enumerate(fs):
+        batch.set_fields(*f, sys=i)
+    batch.calc_poly_density()
+    batch.iterate(2)
+    Hb = batch.calc_H()
+    assert np.all(np.isfinite(Hb))
+    batch.close()
+    for i in (0, 1, 2, 46):
+        say(" single", i)
+        one = scft_amd.Context(ps[i], precision=PREC_FP32)
+        one.set_fields(*fs[i])
+        one.calc_poly_density()
+        one.iterate(2)
+        H = one.calc_H()[0]
+        assert abs(H - Hb[i]) < 1e-6 * abs(H), (i, H, Hb[i])
+        one.close()
+
+
 # ------------------------------------------------------ trajectories (golden)
 def _golden(name):
     here = os.path.join(os.path.dirname(__file__), "golden", name)
