@@ -155,6 +155,7 @@ struct scft_ctx {
   std::vector<void*> scratch;                       // per system FP64 scratch
   hipGraphExec_t graph = nullptr;
   int graph_scheme = -1;
+  bool any_cl = false;
 };
 
 namespace {
@@ -665,6 +666,7 @@ int setup_system(scft_ctx* c, int si) {
   if (p.do_CL) {
     ALLOC(etap, 2 * m16);
     ALLOC(etam, m16);
+    ALLOC(rng_ctr, sizeof(unsigned long long));
   }
   ALLOC(red, scft::RED_NSLOTS * sizeof(double2));
   d.nblk = std::min(1024, (M + 255) / 256);
@@ -693,6 +695,19 @@ int setup_system(scft_ctx* c, int si) {
   d.lam_imag = p.lam_imag;
   d.lam_real = p.lam_real;
   d.fA = fD;
+  d.seed = p.seed;
+  {
+    // generate_1s_noise scales, code/1s_update.cpp:360-362; the reference
+    // draws the w+ and AB- noise with lam_imag and the AB+ noise with
+    // lam_real (code/1s_update.cpp:55, 91-92)
+    const double lams[3] = {p.lam_imag, p.lam_real, p.lam_imag};
+    for (int f = 0; f < 3; f++) {
+      double sc = std::sqrt(2.0 * lams[f]);
+      for (int dd = 0; dd < p.dim; dd++) sc /= std::sqrt(s.dx[dd]);
+      d.noise_scale[f] = sc;
+    }
+  }
+  if (p.do_CL) HIPCK(hipMemset(d.rng_ctr, 0, sizeof(unsigned long long)));
   HIPCK(hipMemcpy(d.rho_np, rho_np.data(), m16, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(d.hhat_k, hk.data(), m8, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(d.Xk, X.data(), m8, hipMemcpyHostToDevice));
@@ -770,7 +785,7 @@ JobSpec job(int sys, int prec, void* in, void* out, int gkind, double scale) {
 
 int build_programs(scft_ctx* c) {
   const int ns = (int)c->sys.size();
-  std::vector<JobSpec> smear, prop, fwd, inv;
+  std::vector<JobSpec> smear, prop, fwd, inv, noise;
   for (int i = 0; i < ns; i++) {
     SysHost& s = c->sys[i];
     SysDev& d = s.dev;
@@ -798,12 +813,19 @@ int build_programs(scft_ctx* c) {
     for (void* a : {(void*)d.rhoA, (void*)d.rhoB, (void*)d.wpl, (void*)d.wabm, (void*)d.wabp})
       fwd.push_back(job(i, 0, a, a, 0, inv_m));
     for (void* a : {(void*)d.wpl, (void*)d.wabm, (void*)d.wabp}) inv.push_back(job(i, 0, a, a, 0, 1.0));
+    if (d.etap) {
+      noise.push_back(job(i, 0, d.etap, d.etap, 0, inv_m));
+      noise.push_back(job(i, 0, (char*)d.etap + (size_t)s.M * 16, (char*)d.etap + (size_t)s.M * 16, 0, inv_m));
+      noise.push_back(job(i, 0, d.etam, d.etam, 0, inv_m));
+    }
   }
   int rc;
   if ((rc = build_program(c, scft::MODE_CONV, smear, &c->prog_smear))) return rc;
   if ((rc = build_program(c, scft::MODE_CONV, prop, &c->prog_prop))) return rc;
   if ((rc = build_program(c, scft::MODE_FWD, fwd, &c->prog_fwd))) return rc;
   if ((rc = build_program(c, scft::MODE_INV, inv, &c->prog_inv))) return rc;
+  if (!noise.empty() && (rc = build_program(c, scft::MODE_FWD, noise, &c->prog_noise))) return rc;
+  c->any_cl = !noise.empty();
   c->prog_fft_fwd.resize(ns);
   c->prog_fft_inv.resize(ns);
   for (int i = 0; i < ns; i++) {
@@ -835,7 +857,14 @@ int stage_poly_density(scft_ctx* c) {
 }
 
 int stage_update(scft_ctx* c, int scheme) {
-  int rc = run_program(c, c->prog_fwd, 0);
+  int rc;
+  if (c->any_cl) {
+    // noise for this iteration (real space), then to k-space like the fields
+    HIPCK(scft::launch_noise(c->d_sys, (int)c->sys.size(), c->maxM, c->st));
+    if ((rc = run_program(c, c->prog_noise, 0))) return rc;
+    HIPCK(scft::launch_noise_advance(c->d_sys, (int)c->sys.size(), c->st));
+  }
+  rc = run_program(c, c->prog_fwd, 0);
   if (rc) return rc;
   HIPCK(scft::launch_update(scheme, c->d_sys, (int)c->sys.size(), c->maxM, c->st));
   rc = run_program(c, c->prog_inv, 0);
@@ -1060,6 +1089,18 @@ int scft_get_Q(scft_ctx* c, int sys, double* Q) {
   return SCFT_OK;
 }
 
+int scft_get_noise(scft_ctx* c, int sys, double* a, double* b, double* m) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  SysHost& s = c->sys[sys];
+  if (!s.dev.etap) return fail(SCFT_EINVAL, "system has no Langevin noise (do_CL = 0)");
+  const size_t bytes = (size_t)s.M * 16;
+  if (a && (rc = copy_out(c, a, s.dev.etap, bytes))) return rc;
+  if (b && (rc = copy_out(c, b, (char*)s.dev.etap + bytes, bytes))) return rc;
+  if (m && (rc = copy_out(c, m, s.dev.etam, bytes))) return rc;
+  return SCFT_OK;
+}
+
 int scft_fft(scft_ctx* c, int sys, double* inout, int dir) {
   int rc = check_ctx(c, sys);
   if (rc) return rc;
@@ -1093,7 +1134,6 @@ int scft_update_1s(scft_ctx* c) {
 int scft_update_euler(scft_ctx* c) {
   int rc = check_ctx(c, 0);
   if (rc) return rc;
-  if (c->sys[0].p.do_CL) return fail(SCFT_EUNSUPPORTED, "complex-Langevin noise is not built yet");
   rc = stage_update(c, 0);
   if (rc) return rc;
   HIPCK(hipStreamSynchronize(c->st));
@@ -1234,7 +1274,6 @@ int scft_iterate(scft_ctx* c, int n_iter) {
   if (rc) return rc;
   if (n_iter < 0) return fail(SCFT_EINVAL, "negative iteration count");
   const int scheme = c->sys[0].p.update_scheme;
-  if (c->sys[0].p.do_CL) return fail(SCFT_EUNSUPPORTED, "complex-Langevin noise is not built yet");
   if (!c->graph || c->graph_scheme != scheme) {
     // the first iteration runs eagerly (sets kernel attributes), the
     // second is captured

@@ -51,7 +51,10 @@ struct SysDev {
   double *X2k;              // phiD rho0 N h^2 (gaa - gbb), negative-chi branch
   void *q, *qd;             // N * M propagators, propagator precision
   void *expA, *expB;        // exp(-(smw - shift)), propagator precision
-  double2 *etap, *etam;     // noise (k-space)
+  double2 *etap, *etam;     // noise (k-space): etap[0..M) w+, etap[M..2M) AB+, etam AB-
+  unsigned long long *rng_ctr;  // iteration counter of the noise generator
+  unsigned long long seed;
+  double noise_scale[3];    // sqrt(2 lambda) / prod sqrt(dx) of the three fields
   double2 *red;             // scalar slots, see RED_*
   double2 *partials;        // reduction scratch, nblk per slot
   int nblk;                 // partial blocks of a full-grid reduction

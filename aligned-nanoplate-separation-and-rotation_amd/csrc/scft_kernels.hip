@@ -444,6 +444,64 @@ __global__ __launch_bounds__(256) void k_update_euler(const SysDev* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Complex-Langevin noise, generate_1s_noise (code/1s_update.cpp:356-368):
+// real Gaussian noise of scale sqrt(2 lambda) / prod_d sqrt(dx_d) in real
+// space (forward-transformed afterwards).  Counter-based Philox4x32-10
+// (Salmon et al., SC'11) keyed by the system's seed; counter = (point,
+// field, iteration): every point draws independently, a replayed graph gets
+// a new iteration number from the device-side counter, and a system's noise
+// does not depend on its place in a batch.  Box-Muller on two 53-bit
+// uniforms.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox_round(uint32_t c[4], uint32_t k[2]) {
+  const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+  const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+  const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+  const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+  const uint32_t n0 = hi1 ^ c[1] ^ k[0], n2 = hi0 ^ c[3] ^ k[1];
+  c[0] = n0;
+  c[1] = lo1;
+  c[2] = n2;
+  c[3] = lo0;
+}
+
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k[2]) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    philox_round(c, k);
+    if (r < 9) {
+      k[0] += 0x9E3779B9u;
+      k[1] += 0xBB67AE85u;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_noise(const SysDev* __restrict__ sys) {
+  const SysDev& s = sys[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (!s.do_cl || i >= s.M) return;
+  const unsigned long long it = *s.rng_ctr;
+#pragma unroll
+  for (int f = 0; f < 3; f++) {
+    uint32_t c[4] = {(uint32_t)i, (uint32_t)f, (uint32_t)it, (uint32_t)(it >> 32)};
+    uint32_t k[2] = {(uint32_t)s.seed, (uint32_t)(s.seed >> 32)};
+    philox4x32_10(c, k);
+    const double u1 = ((double)((((uint64_t)c[0]) << 21) ^ (uint64_t)c[1]) + 0.5) * 0x1p-53;
+    const double u2 = ((double)((((uint64_t)c[2]) << 21) ^ (uint64_t)c[3]) + 0.5) * 0x1p-53;
+    const double z = sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+    const double2 v = make_double2(s.noise_scale[f] * z, 0.0);
+    if (f == 0) s.etap[i] = v;
+    else if (f == 1) s.etap[s.M + i] = v;
+    else s.etam[i] = v;
+  }
+}
+
+__global__ void k_noise_advance(const SysDev* __restrict__ sys, int nsys) {
+  const int j = threadIdx.x + blockIdx.x * blockDim.x;
+  if (j < nsys && sys[j].do_cl) *sys[j].rng_ctr += 1;
+}
+
+// ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
 template <typename R, int MODE, int TB, bool PLANE>
@@ -519,6 +577,16 @@ hipError_t launch_exp_prop(int prec, const SysDev* s, int nsys, int maxM, hipStr
 hipError_t launch_density(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st) {
   if (prec == 0) hipLaunchKernelGGL(k_density<double>, pgrid(maxM, nsys), dim3(256), 0, st, s);
   else hipLaunchKernelGGL(k_density<float>, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_noise(const SysDev* s, int nsys, int maxM, hipStream_t st) {
+  hipLaunchKernelGGL(k_noise, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_noise_advance(const SysDev* s, int nsys, hipStream_t st) {
+  hipLaunchKernelGGL(k_noise_advance, dim3((nsys + 63) / 64), dim3(64), 0, st, s, nsys);
   return hipGetLastError();
 }
 

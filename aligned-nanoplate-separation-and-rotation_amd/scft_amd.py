@@ -294,6 +294,7 @@ def load_library(path: str | None = None):
         "scft_get_cavity": (ctypes.c_int, [vp, ctypes.c_int, dp]),
         "scft_get_Q": (ctypes.c_int, [vp, ctypes.c_int, dp]),
         "scft_fft": (ctypes.c_int, [vp, ctypes.c_int, dp, ctypes.c_int]),
+        "scft_get_noise": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp]),
         "scft_calc_poly_density": (ctypes.c_int, [vp]),
         "scft_update_1s": (ctypes.c_int, [vp]),
         "scft_update_euler": (ctypes.c_int, [vp]),
@@ -401,6 +402,11 @@ class Context:
         out = np.empty(2)
         self._check(self.lib.scft_get_Q(self.h, sys, _dptr(out)), "scft_get_Q")
         return complex(out[0], out[1])
+
+    def get_noise(self, sys=0):
+        out = [np.empty(self.M(sys), np.complex128) for _ in range(3)]
+        self._check(self.lib.scft_get_noise(self.h, sys, *[_dptr(x) for x in out]), "scft_get_noise")
+        return out
 
     def fft(self, a, direction: int, sys=0):
         x = np.array(a, dtype=np.complex128, copy=True)

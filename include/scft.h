@@ -133,6 +133,13 @@ int scft_get_cavity(scft_ctx* ctx, int sys, double* rho);
 /* Single-chain partition function Qd of the last density evaluation. */
 int scft_get_Q(scft_ctx* ctx, int sys, double* Q_re_im);
 
+/* Complex-Langevin noise of the last update, in k-space (the reference's
+ * etap / etam globals, code/globals.h:111, filled by generate_1s_noise,
+ * code/1s_update.cpp:356-368): the w+ noise, the AB+ noise and the AB- noise.
+ * SCFT_EINVAL if the system has do_CL = 0. */
+int scft_get_noise(scft_ctx* ctx, int sys, double* eta_wpl, double* eta_abp,
+                   double* eta_abm);
+
 /* FFT seam, fft_fwd_wrapper / fft_bck_wrapper (code/fft_wrappers.cpp:4-67):
  * dir = -1 forward, scaled by 1/M; dir = +1 backward, unscaled.  `inout` is a
  * host array of M interleaved complex values of system `sys`'s grid. */

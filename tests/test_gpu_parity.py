@@ -261,6 +261,84 @@ def test_config3_sweep_one_launch():
         one.close()
 
 
+# ------------------------------------------------------- complex Langevin
+def _cl_workload(**kw):
+    base = dict(grid=(16, 16, 32), box_rg=(2.0, 2.0, 4.0), N=16, C=5.0, update_scheme=0, do_CL=1,
+                dt=0.001, seed=77)
+    base.update(kw)
+    return Workload(**base)
+
+
+def test_cl_noise_statistics():
+    # generate_1s_noise (code/1s_update.cpp:356-368): real Gaussian noise of
+    # variance 2 lambda / prod dx in real space; the w+ and AB- fields use
+    # lam_imag, the AB+ field lam_real
+    say("cl noise statistics")
+    p = _cl_workload().params()
+    p.lam_real = 2.0 * p.lam_imag   # make the swapped scale visible
+    ctx = scft_amd.Context(p)
+    ctx.calc_poly_density()
+    ctx.update_euler()
+    eta = ctx.get_noise()
+    dV = np.prod([p.L[d] / p.Nx[d] for d in range(3)])
+    lams = (p.lam_imag, p.lam_real, p.lam_imag)
+    for e, lam in zip(eta, lams):
+        x = ctx.fft(e, +1)               # back to real space
+        assert np.max(np.abs(x.imag)) < 1e-12 * np.max(np.abs(x.real))
+        var = 2.0 * lam / dV
+        M = p.M
+        assert abs(np.mean(x.real)) < 6 * np.sqrt(var / M)
+        assert abs(np.var(x.real) / var - 1.0) < 6 * np.sqrt(2.0 / M)
+    ctx.close()
+
+
+def test_cl_noise_fresh_and_reproducible():
+    say("cl noise reproducible")
+    p = _cl_workload().params()
+    a = scft_amd.Context(p)
+    b = scft_amd.Context(p)
+    for c in (a, b):
+        c.calc_poly_density()
+        c.update_euler()
+    ea, eb = a.get_noise()[0], b.get_noise()[0]
+    assert np.array_equal(ea, eb)                     # same seed, same draw
+    a.update_euler()
+    assert not np.allclose(a.get_noise()[0], ea)      # next iteration, new draw
+    p2 = _cl_workload(seed=78).params()
+    c = scft_amd.Context(p2)
+    c.calc_poly_density()
+    c.update_euler()
+    assert np.corrcoef(c.get_noise()[0].real, ea.real)[0, 1] < 0.1
+    for x in (a, b, c):
+        x.close()
+
+
+def test_cl_fp32_tracks_fp64_and_replicas():
+    # identical noise streams: the FP32-propagator run follows the FP64 run;
+    # replicas with other seeds in the same context follow their own runs
+    say("cl fp32 vs fp64")
+    ws = [_cl_workload(seed=100 + r) for r in range(4)]
+    ps = [w.params() for w in ws]
+    f = seeded_fields(ps[0], 3, period_rg=2.0, amp=0.5, sigma=0.01)
+    c64 = scft_amd.Context(ps, precision=PREC_FP64)
+    c32 = scft_amd.Context(ps, precision=PREC_FP32)
+    for c in (c64, c32):
+        for r in range(4):
+            c.set_fields(*f, sys=r)
+        c.calc_poly_density()
+        c.iterate(20)
+    H64, H32 = c64.calc_H(), c32.calc_H()
+    assert np.all(np.abs(H32 - H64) < 1e-5 * np.abs(H64)), (H32, H64)
+    assert len(set(np.round(H64.real, 6))) == 4      # replicas differ
+    one = scft_amd.Context(ps[2], precision=PREC_FP64)
+    one.set_fields(*f)
+    one.calc_poly_density()
+    one.iterate(20)
+    assert abs(one.calc_H()[0] - H64[2]) < 1e-11 * abs(H64[2])
+    for c in (c64, c32, one):
+        c.close()
+
+
 # ------------------------------------------------------ trajectories (golden)
 def _golden(name):
     here = os.path.join(os.path.dirname(__file__), "golden", name)
