@@ -24,6 +24,10 @@ hipError_t launch_copy16(const void* src, void* dst, long n16, hipStream_t st);
 hipError_t launch_assemble(const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_exp_prop(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_density(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);
+hipError_t launch_cavity(const SysDev* s, int nsys, int maxM, hipStream_t st);
+hipError_t launch_plate_grad(const SysDev* one, int nblk, const double* axis, double lq_re, double lq_im,
+                             double2* out, hipStream_t st);
+hipError_t launch_scale_real(double* a, long n, double f, hipStream_t st);
 hipError_t launch_noise(const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_noise_advance(const SysDev* s, int nsys, hipStream_t st);
 hipError_t launch_update(int scheme, const SysDev* s, int nsys, int maxM, hipStream_t st);

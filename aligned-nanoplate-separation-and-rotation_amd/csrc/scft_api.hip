@@ -135,6 +135,7 @@ struct SysHost {
   // 1D tables, per axis: bond (prop precision) and smear (FP64), device
   void* bond_g[3];
   void* smear_g[3];
+  double2* grad_out = nullptr;  // device, 8 complex
   std::vector<void*> allocs;
 };
 
@@ -491,6 +492,17 @@ struct Ran2 {
   }
 };
 
+// Device row of plate np from the host parameters (absolute centre, axes).
+void set_plate_row(SysHost& s, int np) {
+  scft::PlateDev& P = s.dev.plate[np];
+  const scft_params& p = s.p;
+  for (int d = 0; d < 3; d++) {
+    P.c[d] = d < p.dim ? p.exp_nr_c[np][d] * s.L[d] : 0.0;
+    P.u[d] = d < p.dim ? p.exp_nr_u[np][d] : 0.0;
+  }
+  plate_axes(P.u, P.e1, P.e2);
+}
+
 int setup_system(scft_ctx* c, int si) {
   SysHost& s = c->sys[si];
   const scft_params& p = s.p;
@@ -671,6 +683,20 @@ int setup_system(scft_ctx* c, int si) {
   ALLOC(red, scft::RED_NSLOTS * sizeof(double2));
   d.nblk = std::min(1024, (M + 255) / 256);
   ALLOC(partials, (size_t)scft::RED_NSLOTS * d.nblk * sizeof(double2));
+  ALLOC(grad_partials, (size_t)8 * d.nblk * sizeof(double2));
+  rc = dalloc(c, s.allocs, &p_, 8 * sizeof(double2));
+  if (rc) return rc;
+  s.grad_out = (double2*)p_;
+  d.n_plates = std::min(p.n_exp_nr, 2);
+  d.np_type = p.np_type;
+  for (int k = 0; k < 3; k++) {
+    d.Lb[k] = s.L[k];
+    d.dxb[k] = s.dx[k];
+  }
+  d.len_nr = p.L_nr;
+  d.R_nr = p.R_nr;
+  d.xi_nr = p.xi_nr;
+  for (int np = 0; np < 2; np++) set_plate_row(s, np);
   rc = dalloc(c, s.allocs, &p_, m16);
   if (rc) return rc;
   c->scratch[si] = p_;
@@ -1098,6 +1124,72 @@ int scft_get_noise(scft_ctx* c, int sys, double* a, double* b, double* m) {
   if (a && (rc = copy_out(c, a, s.dev.etap, bytes))) return rc;
   if (b && (rc = copy_out(c, b, (char*)s.dev.etap + bytes, bytes))) return rc;
   if (m && (rc = copy_out(c, m, s.dev.etam, bytes))) return rc;
+  return SCFT_OK;
+}
+
+int scft_set_plates(scft_ctx* c, int sys, const double* centres, const double* u) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  SysHost& s = c->sys[sys];
+  const int np = s.dev.n_plates;
+  if (np == 0) return fail(SCFT_EINVAL, "system has no plates");
+  for (int j = 0; j < np; j++) {
+    double m2 = 0.0;
+    for (int d = 0; d < s.p.dim; d++) m2 += u[3 * j + d] * u[3 * j + d];
+    if (m2 <= 0.0) return fail(SCFT_EINVAL, "plate orientation must be nonzero");
+    for (int d = 0; d < s.p.dim; d++) {
+      if (centres[3 * j + d] < 0.0 || centres[3 * j + d] > 1.0)
+        return fail(SCFT_EINVAL, "plate centres are fractions of the box (0..1)");
+      s.p.exp_nr_c[j][d] = centres[3 * j + d];
+      s.p.exp_nr_u[j][d] = u[3 * j + d] / std::sqrt(m2);
+    }
+    set_plate_row(s, j);
+  }
+  SysDev* row = c->d_sys + sys;
+  HIPCK(hipMemcpyAsync(row, &s.dev, sizeof(SysDev), hipMemcpyHostToDevice, c->st));
+  HIPCK(scft::launch_cavity(row, 1, s.M, c->st));
+  // exp_nrH = forward transform of the plate density
+  HIPCK(hipMemcpyAsync(c->scratch[sys], s.dev.rho_np, (size_t)s.M * 16, hipMemcpyDeviceToDevice, c->st));
+  if ((rc = run_program(c, c->prog_fft_fwd[sys], 0))) return rc;
+  HIPCK(hipMemcpyAsync(s.dev.exp_nrH, c->scratch[sys], (size_t)s.M * 16, hipMemcpyDeviceToDevice, c->st));
+  // volume bookkeeping, code/initialize.cpp:249-355
+  HIPCK(scft::launch_reduce(scft::RED_VEXP, 0, row, 1, s.dev.nblk, c->st));
+  double2 red[scft::RED_NSLOTS];
+  if ((rc = copy_out(c, red, s.dev.red, sizeof red))) return rc;
+  const double V_exp = red[scft::RED_VEXP].x * s.dV;
+  scft_sysinfo& inf = s.info;
+  const double phiD_old = inf.phiD;
+  inf.np_frac = V_exp / inf.Vf;
+  inf.V_1_exp_np = V_exp / double(s.p.n_exp_nr);
+  inf.phiD = 1.0 - s.p.phiH - inf.np_frac - inf.phiG;
+  inf.nD = inf.phiD * s.p.rho0 * inf.Vf / s.N;
+  s.dev.nD = inf.nD;
+  if (phiD_old != 0.0) {
+    // X(k) and X2(k) carry phiD as a factor
+    HIPCK(scft::launch_scale_real(s.dev.Xk, s.M, inf.phiD / phiD_old, c->st));
+    HIPCK(scft::launch_scale_real(s.dev.X2k, s.M, inf.phiD / phiD_old, c->st));
+  }
+  HIPCK(hipMemcpyAsync(row, &s.dev, sizeof(SysDev), hipMemcpyHostToDevice, c->st));
+  HIPCK(hipStreamSynchronize(c->st));
+  return SCFT_OK;
+}
+
+int scft_grad_plates(scft_ctx* c, int sys, const double* axis, double* out) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  if (!axis || !out) return fail(SCFT_EINVAL, "null argument");
+  SysHost& s = c->sys[sys];
+  double2 red[scft::RED_NSLOTS];
+  if ((rc = copy_out(c, red, s.dev.red, sizeof red))) return rc;
+  const cd Qs = cd(red[scft::RED_Q].x, red[scft::RED_Q].y) * (s.dV / s.info.V);
+  const cd logQ = std::log(Qs) - double(s.N) * red[scft::RED_SHIFT].y;
+  HIPCK(scft::launch_plate_grad(c->d_sys + sys, s.dev.nblk, axis, logQ.real(), logQ.imag(), s.grad_out, c->st));
+  double2 g[8];
+  if ((rc = copy_out(c, g, s.grad_out, sizeof g))) return rc;
+  for (int k = 0; k < 8; k++) {
+    out[2 * k] = k < 4 * s.dev.n_plates ? g[k].x : 0.0;
+    out[2 * k + 1] = k < 4 * s.dev.n_plates ? g[k].y : 0.0;
+  }
   return SCFT_OK;
 }
 

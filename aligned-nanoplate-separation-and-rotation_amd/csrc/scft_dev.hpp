@@ -35,6 +35,12 @@ struct PassDesc {
   double scale;
 };
 
+// One plate: absolute centre and body axes (u normal, e1 e2 in-plane).
+struct PlateDev {
+  double c[3];
+  double u[3], e1[3], e2[3];
+};
+
 // Per-system state for the pointwise kernels.
 struct SysDev {
   int M, N, Na, nx, ny, nz;
@@ -58,6 +64,12 @@ struct SysDev {
   double2 *red;             // scalar slots, see RED_*
   double2 *partials;        // reduction scratch, nblk per slot
   int nblk;                 // partial blocks of a full-grid reduction
+  // plates (code/initialize.cpp:578-612 and the square extension)
+  int n_plates, np_type;
+  double Lb[3], dxb[3];
+  double len_nr, R_nr, xi_nr;
+  PlateDev plate[2];
+  double2 *grad_partials;   // 8 complex per block: plate derivatives of H
 };
 
 enum {
@@ -65,7 +77,8 @@ enum {
   RED_H = 1,       // sum of the field part of H
   RED_SHIFT = 2,   // sum of Re(fA smwa + (1 - fA) smwb); .y = shift c
   RED_RHO = 3,     // sum of rhoA + rhoB (segment count check)
-  RED_NSLOTS = 4
+  RED_VEXP = 4,    // sum of the plate density (volume bookkeeping)
+  RED_NSLOTS = 5
 };
 
 }  // namespace scft

@@ -11,6 +11,7 @@
 // Pointwise kernels take a device table of SysDev, blockIdx.y = system.
 #include <hip/hip_runtime.h>
 
+#include "../../include/scft.h"
 #include "kernels.h"
 #include "scft_dev.hpp"
 
@@ -263,6 +264,8 @@ __device__ __forceinline__ double2 red_term(const SysDev& s, long i) {
     return t;
   } else if constexpr (WHAT == RED_SHIFT) {
     return make_double2(s.fA * s.wa[i].x + (1.0 - s.fA) * s.wb[i].x, 0.0);
+  } else if constexpr (WHAT == RED_VEXP) {
+    return s.rho_np[i];
   } else {
     return dadd(s.rhoA[i], s.rhoB[i]);
   }
@@ -502,6 +505,154 @@ __global__ void k_noise_advance(const SysDev* __restrict__ sys, int nsys) {
 }
 
 // ---------------------------------------------------------------------------
+// Plate density and its derivatives.  Rod (np_type 2, explicit_nanorod,
+// code/initialize.cpp:578-612): 1/4 erfc(a) erfc(b), a = (|u.dr| - len/2)/xi,
+// b = (|u x dr| - R)/xi, dr the minimum image of x - c (x = n dx).  Square
+// plate (np_type 3, 3D): 1/8 erfc(a) erfc(b1) erfc(b2), b_i = (|e_i.dr| - R)/xi.
+// Derivatives are those of the sampled formula: d/dc (dr -> -1) and the
+// rotation of the plate's axes about `axis` through its centre
+// (d v / d theta = axis x v).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dot3(const double* a, const double* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+__device__ __forceinline__ void cross3(const double* a, const double* b, double* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+// d erfc(z) / dz
+__device__ __forceinline__ double derfc(double z) { return -1.1283791670955125739 * exp(-z * z); }
+
+template <bool GRAD>
+__device__ __forceinline__ double plate_eval(const SysDev& s, int p, long i, const double* axis, double* dc,
+                                             double* dth) {
+  const int nn[3] = {(int)(i % s.nx), (int)((i / s.nx) % s.ny), (int)(i / ((long)s.nx * s.ny))};
+  const PlateDev& P = s.plate[p];
+  double dr[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < s.dim; d++) {
+    dr[d] = (double)nn[d] * s.dxb[d] - P.c[d];
+    if (dr[d] >= 0.5 * s.Lb[d]) dr[d] -= s.Lb[d];
+    else if (dr[d] < -0.5 * s.Lb[d]) dr[d] += s.Lb[d];
+  }
+  const double xi = s.xi_nr;
+  const double ud = dot3(P.u, dr);
+  const double a = (fabs(ud) - 0.5 * s.len_nr) / xi;
+  const double ea = erfc(a);
+  const double sud = ud > 0.0 ? 1.0 : (ud < 0.0 ? -1.0 : 0.0);
+  double du[3] = {0.0, 0.0, 0.0};
+  if (GRAD) cross3(axis, P.u, du);
+  if (s.np_type == SCFT_NP_SQUARE && s.dim == 3) {
+    const double a1 = dot3(P.e1, dr), a2 = dot3(P.e2, dr);
+    const double b1 = (fabs(a1) - s.R_nr) / xi, b2 = (fabs(a2) - s.R_nr) / xi;
+    const double e1v = erfc(b1), e2v = erfc(b2);
+    if (GRAD) {
+      const double s1 = a1 > 0.0 ? 1.0 : (a1 < 0.0 ? -1.0 : 0.0);
+      const double s2 = a2 > 0.0 ? 1.0 : (a2 < 0.0 ? -1.0 : 0.0);
+      const double ga = 0.125 * derfc(a) * e1v * e2v / xi;
+      const double g1 = 0.125 * ea * derfc(b1) * e2v / xi;
+      const double g2 = 0.125 * ea * e1v * derfc(b2) / xi;
+      double de1[3], de2[3];
+      cross3(axis, P.e1, de1);
+      cross3(axis, P.e2, de2);
+      for (int k = 0; k < 3; k++) dc[k] = -(ga * sud * P.u[k] + g1 * s1 * P.e1[k] + g2 * s2 * P.e2[k]);
+      *dth = ga * sud * dot3(du, dr) + g1 * s1 * dot3(de1, dr) + g2 * s2 * dot3(de2, dr);
+    }
+    return 0.125 * ea * e1v * e2v;
+  }
+  double sc, dsc_dr[3] = {0.0, 0.0, 0.0}, dsc_dth = 0.0;
+  if (s.dim == 2) {
+    const double cz = P.u[0] * dr[1] - P.u[1] * dr[0];
+    sc = fabs(cz);
+    if (GRAD) {
+      const double sg = cz > 0.0 ? 1.0 : (cz < 0.0 ? -1.0 : 0.0);
+      dsc_dr[0] = -sg * P.u[1];
+      dsc_dr[1] = sg * P.u[0];
+      dsc_dth = sg * (du[0] * dr[1] - du[1] * dr[0]);
+    }
+  } else {
+    double cr[3];
+    cross3(P.u, dr, cr);
+    sc = sqrt(dot3(cr, cr));
+    if (GRAD && sc > 0.0) {
+      for (int k = 0; k < 3; k++) dsc_dr[k] = (dr[k] - ud * P.u[k]) / sc;
+      dsc_dth = -ud * dot3(du, dr) / sc;
+    }
+  }
+  const double b = (sc - s.R_nr) / xi;
+  const double eb = erfc(b);
+  if (GRAD) {
+    const double ga = 0.25 * derfc(a) * eb / xi;
+    const double gb = 0.25 * ea * derfc(b) / xi;
+    for (int k = 0; k < 3; k++) dc[k] = -(ga * sud * P.u[k] + gb * dsc_dr[k]);
+    *dth = ga * sud * dot3(du, dr) + gb * dsc_dth;
+  }
+  return 0.25 * ea * eb;
+}
+
+__global__ __launch_bounds__(256) void k_cavity(const SysDev* __restrict__ sys) {
+  const SysDev& s = sys[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.M) return;
+  double rho = 0.0;
+  for (int p = 0; p < s.n_plates; p++) rho += plate_eval<false>(s, p, i, nullptr, nullptr, nullptr);
+  s.rho_np[i] = make_double2(rho, 0.0);
+}
+
+// sum over points of d rho_p / d g times W = i rho0 w+ + (rho0 / N) log Q,
+// times dV: the explicit derivative of H (code/calc_h.cpp:71-72 through
+// rho_np, and -nD log Q through nD = rho0 (Vf - V_exp) / N)
+__global__ __launch_bounds__(256) void k_plate_grad1(const SysDev* __restrict__ sysp, double ax, double ay,
+                                                     double az, double lq_re, double lq_im) {
+  const SysDev& s = *sysp;
+  const double axis[3] = {ax, ay, az};
+  __shared__ double2 sh[8][256];
+  double2 acc[8];
+  for (int k = 0; k < 8; k++) acc[k] = make_double2(0.0, 0.0);
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < s.M; i += (long)s.nblk * 256) {
+    const double2 w = s.wpl[i];
+    // W = i rho0 w + (rho0 / N) log Q
+    const double2 W = make_double2(-s.rho0 * w.y + s.rho0 / s.N * lq_re, s.rho0 * w.x + s.rho0 / s.N * lq_im);
+    for (int p = 0; p < s.n_plates; p++) {
+      double dc[3], dth;
+      plate_eval<true>(s, p, i, axis, dc, &dth);
+      for (int k = 0; k < 3; k++) acc[4 * p + k] = dadd(acc[4 * p + k], dscale(W, dc[k]));
+      acc[4 * p + 3] = dadd(acc[4 * p + 3], dscale(W, dth));
+    }
+  }
+  for (int k = 0; k < 8; k++) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 8; k++) sh[k][threadIdx.x] = dadd(sh[k][threadIdx.x], sh[k][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) s.grad_partials[(long)threadIdx.x * s.nblk + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_plate_grad2(const SysDev* __restrict__ sysp, double2* out) {
+  const SysDev& s = *sysp;
+  __shared__ double2 sh[256];
+  for (int k = 0; k < 8; k++) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int b = threadIdx.x; b < s.nblk; b += 256) acc = dadd(acc, s.grad_partials[(long)k * s.nblk + b]);
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] = dadd(sh[threadIdx.x], sh[threadIdx.x + w]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = dscale(sh[0], s.dV);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scale_real(double* a, long n, double f) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) a[i] *= f;
+}
+
+// ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
 template <typename R, int MODE, int TB, bool PLANE>
@@ -580,6 +731,23 @@ hipError_t launch_density(int prec, const SysDev* s, int nsys, int maxM, hipStre
   return hipGetLastError();
 }
 
+hipError_t launch_cavity(const SysDev* s, int nsys, int maxM, hipStream_t st) {
+  hipLaunchKernelGGL(k_cavity, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_plate_grad(const SysDev* one, int nblk, const double* axis, double lq_re, double lq_im,
+                             double2* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_plate_grad1, dim3(nblk), dim3(256), 0, st, one, axis[0], axis[1], axis[2], lq_re, lq_im);
+  hipLaunchKernelGGL(k_plate_grad2, dim3(1), dim3(256), 0, st, one, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_real(double* a, long n, double f, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale_real, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n, f);
+  return hipGetLastError();
+}
+
 hipError_t launch_noise(const SysDev* s, int nsys, int maxM, hipStream_t st) {
   hipLaunchKernelGGL(k_noise, pgrid(maxM, nsys), dim3(256), 0, st, s);
   return hipGetLastError();
@@ -611,6 +779,10 @@ hipError_t launch_reduce(int what, int prec, const SysDev* s, int nsys, int nblk
     case RED_SHIFT:
       hipLaunchKernelGGL((k_reduce1<RED_SHIFT, double>), g1, dim3(256), 0, st, s);
       hipLaunchKernelGGL(k_reduce2<RED_SHIFT>, g2, dim3(256), 0, st, s);
+      break;
+    case RED_VEXP:
+      hipLaunchKernelGGL((k_reduce1<RED_VEXP, double>), g1, dim3(256), 0, st, s);
+      hipLaunchKernelGGL(k_reduce2<RED_VEXP>, g2, dim3(256), 0, st, s);
       break;
     default:
       hipLaunchKernelGGL((k_reduce1<RED_RHO, double>), g1, dim3(256), 0, st, s);

@@ -295,6 +295,8 @@ def load_library(path: str | None = None):
         "scft_get_Q": (ctypes.c_int, [vp, ctypes.c_int, dp]),
         "scft_fft": (ctypes.c_int, [vp, ctypes.c_int, dp, ctypes.c_int]),
         "scft_get_noise": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp]),
+        "scft_set_plates": (ctypes.c_int, [vp, ctypes.c_int, dp, dp]),
+        "scft_grad_plates": (ctypes.c_int, [vp, ctypes.c_int, dp, dp]),
         "scft_calc_poly_density": (ctypes.c_int, [vp]),
         "scft_update_1s": (ctypes.c_int, [vp]),
         "scft_update_euler": (ctypes.c_int, [vp]),
@@ -402,6 +404,34 @@ class Context:
         out = np.empty(2)
         self._check(self.lib.scft_get_Q(self.h, sys, _dptr(out)), "scft_get_Q")
         return complex(out[0], out[1])
+
+    def set_plates(self, centres, us, sys=0):
+        """centres, us: (n_plates, 3) fractional centres and orientations."""
+        c = np.ascontiguousarray(np.asarray(centres, dtype=np.float64).reshape(-1))
+        u = np.ascontiguousarray(np.asarray(us, dtype=np.float64).reshape(-1))
+        self._check(self.lib.scft_set_plates(self.h, sys, _dptr(c), _dptr(u)), "scft_set_plates")
+
+    def set_geometry(self, d: float, theta_deg: float, sys=0):
+        """Two plates at fractional separation d along x, normals rotated by
+        theta about y (the Workload convention)."""
+        th = math.radians(theta_deg)
+        u = [math.sin(th), 0.0, math.cos(th)]
+        self.set_plates([[0.5 - d / 2, 0.5, 0.5], [0.5 + d / 2, 0.5, 0.5]], [u, u], sys=sys)
+
+    def grad_plates(self, axis=(0.0, 1.0, 0.0), sys=0):
+        """(n, 4) complex: dH/dc_x, dH/dc_y, dH/dc_z (b units), dH/dtheta per plate."""
+        a = np.ascontiguousarray(np.asarray(axis, dtype=np.float64))
+        out = np.empty(16)
+        self._check(self.lib.scft_grad_plates(self.h, sys, _dptr(a), _dptr(out)), "scft_grad_plates")
+        g = out[0::2] + 1j * out[1::2]
+        return g.reshape(2, 4)
+
+    def plate_forces(self, sys=0):
+        """dH/dD and dH/dtheta for the symmetric pair of the Workload
+        convention: D = physical centre separation along x (b units), both
+        plates rotating about y."""
+        g = self.grad_plates((0.0, 1.0, 0.0), sys=sys)
+        return 0.5 * (g[1, 0] - g[0, 0]), g[0, 3] + g[1, 3]
 
     def get_noise(self, sys=0):
         out = [np.empty(self.M(sys), np.complex128) for _ in range(3)]

@@ -133,6 +133,27 @@ int scft_get_cavity(scft_ctx* ctx, int sys, double* rho);
 /* Single-chain partition function Qd of the last density evaluation. */
 int scft_get_Q(scft_ctx* ctx, int sys, double* Q_re_im);
 
+/* Move or rotate the plates of system `sys` in place: `centres` and `u` hold
+ * 3 doubles per plate (fractional centres 0..1, orientation vectors,
+ * normalised here), as the plate lines of bcp.input (code/read_input.cpp:
+ * 205-236).  Recomputes the plate density on the device
+ * (code/initialize.cpp:578-612), its transform exp_nrH and the volume
+ * bookkeeping nD, phi_D (code/initialize.cpp:249-355); fields are kept. */
+int scft_set_plates(scft_ctx* ctx, int sys, const double* centres,
+                    const double* u);
+
+/* Explicit derivatives of H at the current fields with respect to each
+ * plate's centre (absolute coordinates, b units) and to a rotation of the
+ * plate about `axis` (unit vector) through its centre.  H depends on the
+ * plates through rho_np in wpl_part (code/calc_h.cpp:71-72) and through nD
+ * in -nD log Qd (code/initialize.cpp:351):
+ *   dH/dg = sum_r d rho_np/dg (i rho0 w+ + (rho0 / N) log Qd) dV.
+ * At a saddle point (dH/dw = 0) these are the total derivatives: the force
+ * and torque on the plates.  out[8 p + 2 k + {0,1}] = Re, Im of dH/dc_k
+ * (k = 0..2) and of dH/dtheta (k = 3), p = 0..1.  Needs a density
+ * evaluation (Qd) of the current fields. */
+int scft_grad_plates(scft_ctx* ctx, int sys, const double* axis, double* out);
+
 /* Complex-Langevin noise of the last update, in k-space (the reference's
  * etap / etam globals, code/globals.h:111, filled by generate_1s_noise,
  * code/1s_update.cpp:356-368): the w+ noise, the AB+ noise and the AB- noise.

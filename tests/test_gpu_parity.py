@@ -261,6 +261,109 @@ def test_config3_sweep_one_launch():
         one.close()
 
 
+# ------------------------------------------------------------ plate forces
+def test_device_cavity_matches_setup():
+    say("device cavity")
+    for np_type, th in ((scft_amd.NP_SQUARE, 20.0), (scft_amd.NP_ROD, 0.0)):
+        w = Workload(grid=(32, 32, 64), box_rg=(4.0, 4.0, 8.0), N=50, np_type=np_type, theta_deg=th)
+        ctx = scft_amd.Context(w.params())
+        rho0 = ctx.get_cavity()
+        nD0 = ctx.info().nD
+        ctx.set_geometry(w.d, w.theta_deg)
+        assert np.max(np.abs(ctx.get_cavity() - rho0)) < 1e-14
+        assert abs(ctx.info().nD - nD0) < 1e-12 * nD0
+        ctx.close()
+
+
+def _fd_setup():
+    w = Workload(grid=(32, 32, 64), box_rg=(4.0, 4.0, 8.0), N=50, d=0.30, theta_deg=5.0)
+    p = w.params()
+    f = seeded_fields(p, 31, period_rg=4.0, amp=1.0, sigma=0.05)
+    return w, p, f
+
+
+def test_plate_gradient_vs_fd_fixed_fields():
+    # at fixed fields H depends on the plates only through rho_np and nD:
+    # the analytic derivative must match central differences of H on the
+    # same grid (delta D = 1e-3 Rg, delta theta = 0.1 deg), and central
+    # differences of the oracle's H
+    say("plate gradient vs FD")
+    w, p, f = _fd_setup()
+    ctx = scft_amd.Context(p)
+    ctx.set_fields(*f)
+    ctx.calc_poly_density()
+    dHdD, dHdth = ctx.plate_forces()
+    Lx = p.L[0]
+    dD = 1e-3 * w.rg
+    dth = 0.1
+
+    def H_at(d, th):
+        ctx.set_geometry(d, th)
+        return ctx.calc_H()[0]
+
+    def fd_D(h):
+        return (H_at(w.d + h / Lx, w.theta_deg) - H_at(w.d - h / Lx, w.theta_deg)) / (2 * h)
+
+    def fd_th(h):
+        return (H_at(w.d, w.theta_deg + h) - H_at(w.d, w.theta_deg - h)) / (2 * np.radians(h))
+    # central differences at the configuration's steps carry O(h^2 / xi^2)
+    # truncation (xi = 0.05 Rg is below the grid spacing); Richardson
+    # extrapolation from h and h/2 removes it
+    fD, fD2 = fd_D(dD), fd_D(dD / 2)
+    ft, ft2 = fd_th(dth), fd_th(dth / 2)
+    rD, rt = (4 * fD2 - fD) / 3, (4 * ft2 - ft) / 3
+    say(" dH/dD", dHdD, fD, rD, " dH/dth", dHdth, ft, rt)
+    assert abs(dHdD - rD) < 1e-6 * abs(rD)
+    assert abs(dHdth - rt) < 1e-6 * abs(rt)
+    assert abs(dHdD - fD) < 5e-4 * abs(fD)
+    assert abs(dHdth - ft) < 5e-4 * abs(ft)
+    fd_D = fD
+    # the oracle's H at the same fields and displaced plates
+    Ho = []
+    for d, th in ((w.d + dD / Lx, w.theta_deg), (w.d - dD / Lx, w.theta_deg)):
+        w2 = Workload(**{**w.__dict__, "d": d, "theta_deg": th})
+        o = pyoracle.Oracle(w2.params())
+        o.set_fields(*f)
+        o.calc_poly_density()
+        Ho.append(o.calc_H())
+        o.close()
+    fd_oracle = (Ho[0] - Ho[1]) / (2 * dD)
+    # same differences from the independent restatement of H(plates)
+    assert abs(fd_oracle - fd_D) < 1e-9 * abs(fd_oracle)
+    ctx.close()
+
+
+def test_plate_force_at_saddle_point():
+    # at a converged saddle point dH*/dD (with the fields re-converged at
+    # D +- delta) equals the explicit derivative: the envelope theorem the
+    # adjoint relies on.  Three systems (D - delta, D, D + delta) in one batch.
+    say("saddle-point force")
+    base = dict(grid=(16, 16, 32), box_rg=(2.0, 2.0, 4.0), N=16, d=0.45, theta_deg=0.0, dt=0.5)
+    w0 = Workload(**base)
+    Lx = w0.params().L[0]
+    dD = 1e-3 * w0.rg
+    ws = [Workload(**{**base, "d": base["d"] + s * dD / Lx}) for s in (-1, 0, 1)]
+    ps = [x.params() for x in ws]
+    f = seeded_fields(ps[1], 4, period_rg=2.0, amp=1.0, sigma=0.0)
+    ctx = scft_amd.Context(ps)
+    for i in range(3):
+        ctx.set_fields(*f, sys=i)
+    ctx.calc_poly_density()
+    prev = None
+    for it in range(40):
+        ctx.iterate(200)
+        H = ctx.calc_H()
+        if prev is not None and np.max(np.abs(H - prev)) < 1e-11 * np.max(np.abs(H)):
+            break
+        prev = H
+    say(" converged after", (it + 1) * 200, "iterations", H)
+    fd = (H[2] - H[0]) / (2 * dD)
+    dHdD, _ = ctx.plate_forces(sys=1)
+    say(" dH*/dD fd", fd, "explicit", dHdD)
+    assert abs(dHdD.real - fd.real) < 1e-4 * abs(fd.real)
+    ctx.close()
+
+
 # ------------------------------------------------------- complex Langevin
 def _cl_workload(**kw):
     base = dict(grid=(16, 16, 32), box_rg=(2.0, 2.0, 4.0), N=16, C=5.0, update_scheme=0, do_CL=1,
