@@ -16,6 +16,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libscft_amd.so")
+DRIVER = os.path.join(HERE, "scft_run")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -48,8 +49,16 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, OUT)
+    # host driver (simulate() loop and its file formats) over the C ABI
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "host", "scft_run.cpp"), "-o", DRIVER + ".tmp", "-L", HERE, "-lscft_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + r.stderr)
+    os.replace(DRIVER + ".tmp", DRIVER)
     if verbose:
-        print("built", OUT)
+        print("built", OUT, "and", DRIVER)
     return OUT
 
 

@@ -30,6 +30,7 @@ hipError_t launch_plate_grad(const SysDev* one, int nblk, const double* axis, do
 hipError_t launch_scale_real(double* a, long n, double f, hipStream_t st);
 hipError_t launch_noise(const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_noise_advance(const SysDev* s, int nsys, hipStream_t st);
+hipError_t launch_cl_accumulate(const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_update(int scheme, const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_reduce(int what, int prec, const SysDev* s, int nsys, int nblk, hipStream_t st);
 

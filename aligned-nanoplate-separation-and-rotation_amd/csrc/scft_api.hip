@@ -514,6 +514,7 @@ int setup_system(scft_ctx* c, int si) {
     return fail(SCFT_EUNSUPPORTED, "grafted chains are not on this path yet");
   if (p.chi_bc != 0.0 || p.chi_ac != 0.0)
     return fail(SCFT_EUNSUPPORTED, "BC/AC field pairs carry graft densities only and are not on this path yet");
+  if (p.do_CL && p.sample_freq < 1) return fail(SCFT_EINVAL, "do_CL needs sample_freq >= 1");
   s.nx = p.Nx[0];
   s.ny = p.Nx[1];
   s.nz = p.dim == 3 ? p.Nx[2] : 1;
@@ -679,6 +680,9 @@ int setup_system(scft_ctx* c, int si) {
     ALLOC(etap, 2 * m16);
     ALLOC(etam, m16);
     ALLOC(rng_ctr, sizeof(unsigned long long));
+    ALLOC(avgA, m16);
+    ALLOC(avgB, m16);
+    ALLOC(n_samples, sizeof(double));
   }
   ALLOC(red, scft::RED_NSLOTS * sizeof(double2));
   d.nblk = std::min(1024, (M + 255) / 256);
@@ -733,7 +737,14 @@ int setup_system(scft_ctx* c, int si) {
       d.noise_scale[f] = sc;
     }
   }
-  if (p.do_CL) HIPCK(hipMemset(d.rng_ctr, 0, sizeof(unsigned long long)));
+  d.sample_wait = p.sample_wait;
+  d.sample_freq = p.sample_freq;
+  if (p.do_CL) {
+    HIPCK(hipMemset(d.rng_ctr, 0, sizeof(unsigned long long)));
+    HIPCK(hipMemset(d.avgA, 0, m16));
+    HIPCK(hipMemset(d.avgB, 0, m16));
+    HIPCK(hipMemset(d.n_samples, 0, sizeof(double)));
+  }
   HIPCK(hipMemcpy(d.rho_np, rho_np.data(), m16, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(d.hhat_k, hk.data(), m8, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(d.Xk, X.data(), m8, hipMemcpyHostToDevice));
@@ -896,6 +907,15 @@ int stage_update(scft_ctx* c, int scheme) {
   rc = run_program(c, c->prog_inv, 0);
   if (rc) return rc;
   return stage_poly_density(c);
+}
+
+// one iteration of the reference main loop (code/simulate.cpp:86-92):
+// update, then the complex-Langevin averages
+int stage_iteration(scft_ctx* c, int scheme) {
+  int rc = stage_update(c, scheme);
+  if (rc) return rc;
+  if (c->any_cl) HIPCK(scft::launch_cl_accumulate(c->d_sys, (int)c->sys.size(), c->maxM, c->st));
+  return SCFT_OK;
 }
 
 int check_ctx(scft_ctx* c, int sys) {
@@ -1124,6 +1144,27 @@ int scft_get_noise(scft_ctx* c, int sys, double* a, double* b, double* m) {
   if (a && (rc = copy_out(c, a, s.dev.etap, bytes))) return rc;
   if (b && (rc = copy_out(c, b, (char*)s.dev.etap + bytes, bytes))) return rc;
   if (m && (rc = copy_out(c, m, s.dev.etam, bytes))) return rc;
+  return SCFT_OK;
+}
+
+int scft_get_averages(scft_ctx* c, int sys, double* avg_a, double* avg_b, double* n_samples) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  SysHost& s = c->sys[sys];
+  if (!s.dev.avgA) return fail(SCFT_EINVAL, "system has no Langevin averages (do_CL = 0)");
+  double n = 0.0;
+  if ((rc = copy_out(c, &n, s.dev.n_samples, sizeof n))) return rc;
+  if (n_samples) *n_samples = n;
+  const size_t bytes = (size_t)s.M * 16;
+  double* outs[2] = {avg_a, avg_b};
+  const void* src[2] = {s.dev.avgA, s.dev.avgB};
+  for (int k = 0; k < 2; k++) {
+    if (!outs[k]) continue;
+    if ((rc = copy_out(c, outs[k], src[k], bytes))) return rc;
+    // write_avg_data_bin divides the sums by n_samples (code/io_utils.cpp:368-369)
+    if (n > 0.0)
+      for (size_t i = 0; i < 2 * (size_t)s.M; i++) outs[k][i] /= n;
+  }
   return SCFT_OK;
 }
 
@@ -1370,7 +1411,7 @@ int scft_iterate(scft_ctx* c, int n_iter) {
     // the first iteration runs eagerly (sets kernel attributes), the
     // second is captured
     if (n_iter == 0) return SCFT_OK;
-    if ((rc = stage_update(c, scheme))) return rc;
+    if ((rc = stage_iteration(c, scheme))) return rc;
     n_iter--;
     if (c->graph) {
       (void)hipGraphExecDestroy(c->graph);
@@ -1378,7 +1419,7 @@ int scft_iterate(scft_ctx* c, int n_iter) {
     }
     hipGraph_t g;
     HIPCK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-    rc = stage_update(c, scheme);
+    rc = stage_iteration(c, scheme);
     hipError_t e = hipStreamEndCapture(c->st, &g);
     if (rc) return rc;
     if (e != hipSuccess) return fail(SCFT_EDEVICE, "graph capture failed");

@@ -60,6 +60,11 @@ struct SysDev {
   double2 *etap, *etam;     // noise (k-space): etap[0..M) w+, etap[M..2M) AB+, etam AB-
   unsigned long long *rng_ctr;  // iteration counter of the noise generator
   unsigned long long seed;
+  // complex-Langevin density averages (code/array_utils.cpp:13-35): running
+  // sums of rhoA, rhoB over the sampled iterations and their count
+  double2 *avgA, *avgB;
+  double *n_samples;
+  int sample_wait, sample_freq;
   double noise_scale[3];    // sqrt(2 lambda) / prod sqrt(dx) of the three fields
   double2 *red;             // scalar slots, see RED_*
   double2 *partials;        // reduction scratch, nblk per slot

@@ -504,6 +504,29 @@ __global__ void k_noise_advance(const SysDev* __restrict__ sys, int nsys) {
   if (j < nsys && sys[j].do_cl) *sys[j].rng_ctr += 1;
 }
 
+// Complex-Langevin averages after the update of iteration it = counter - 1
+// (the counter was advanced when that iteration drew its noise):
+// accumulate when it >= sample_wait and it % sample_freq == 0
+// (code/simulate.cpp:91-92, code/array_utils.cpp:13-35).
+__global__ __launch_bounds__(256) void k_cl_accumulate(const SysDev* __restrict__ sys) {
+  const SysDev& s = sys[blockIdx.y];
+  if (!s.do_cl) return;
+  const long long it = (long long)*s.rng_ctr - 1;
+  if (it < s.sample_wait || it % s.sample_freq != 0) return;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < s.M) {
+    const double2 a = s.rhoA[i], b = s.rhoB[i];
+    double2 sa = s.avgA[i], sb = s.avgB[i];
+    sa.x += a.x;
+    sa.y += a.y;
+    sb.x += b.x;
+    sb.y += b.y;
+    s.avgA[i] = sa;
+    s.avgB[i] = sb;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *s.n_samples += 1.0;
+}
+
 // ---------------------------------------------------------------------------
 // Plate density and its derivatives.  Rod (np_type 2, explicit_nanorod,
 // code/initialize.cpp:578-612): 1/4 erfc(a) erfc(b), a = (|u.dr| - len/2)/xi,
@@ -755,6 +778,11 @@ hipError_t launch_noise(const SysDev* s, int nsys, int maxM, hipStream_t st) {
 
 hipError_t launch_noise_advance(const SysDev* s, int nsys, hipStream_t st) {
   hipLaunchKernelGGL(k_noise_advance, dim3((nsys + 63) / 64), dim3(64), 0, st, s, nsys);
+  return hipGetLastError();
+}
+
+hipError_t launch_cl_accumulate(const SysDev* s, int nsys, int maxM, hipStream_t st) {
+  hipLaunchKernelGGL(k_cl_accumulate, pgrid(maxM, nsys), dim3(256), 0, st, s);
   return hipGetLastError();
 }
 

@@ -265,6 +265,124 @@ def seeded_fields(p: ScftParams, seed: int, period_rg: float = 4.0,
 
 
 # --------------------------------------------------------------------------
+# reference file formats
+# --------------------------------------------------------------------------
+def _num(x) -> str:
+    return repr(float(x)) if isinstance(x, float) else str(x)
+
+
+def write_input(p: ScftParams, path: str) -> None:
+    """Writes `p` as a bcp.input file in the positional layout read_input
+    parses (code/read_input.cpp:31-236; the published inputs, e.g.
+    separation/dist-0.300/bcp.input, show the section order).  Per-dimension
+    lines carry p.dim values; floats are written with full precision."""
+    d = p.dim
+
+    def vec(a):
+        return " ".join(_num(float(a[k])) for k in range(d))
+
+    def ivec(a):
+        return " ".join(str(int(a[k])) for k in range(d))
+
+    L = ["# Main parameters",
+         f"{p.Nda} {p.Ndb}  Na, Nb length of each diblock's block",
+         f"{_num(p.phiH)} {p.Nah}  phiAH, Nah",
+         f"{_num(p.a_smear)}  a, smearing length, 0 for no smearing",
+         f"{_num(p.rho0)}  rho0, bulk density",
+         "",
+         "# Interaction parameters",
+         f"{_num(p.chi_ab)}  chiAB #between diblock",
+         f"{_num(p.chi_bc)}  chiBC #between part B and the graft",
+         f"{_num(p.chi_ac)}  chiAC #between part A and the graft",
+         f"{_num(p.kappa)}   kappa, strength of compressibility potential",
+         "",
+         "#Init field cond flags (0=zero, -1=rand, -2=cosine, -3=sine, -4=factor of 1)"]
+    for k in range(2):
+        L.append(f"{p.ic_flag[k]} {_num(p.ic_pre[k])} {int(p.ic_dir[k])} {_num(p.ic_period[k])}"
+                 f"  flag {k + 1}, prefactor, direction (x=0, y=1, z=2), # periods")
+    L += ["",
+          "# Simulation parameters",
+          f"{ivec(p.Nx)}  Nx, # grid points in each direction",
+          f"{vec(p.L)}    L, box dimension in each direction",
+          f"{_num(p.lam_imag)} lam_pl, step size on the w+ fields",
+          f"{_num(p.lam_real)}  lam_mi, step size on the w- fields",
+          f"{p.itermax}    itermax, max. # of iterations",
+          f"{p.print_freq}       print_freq",
+          f"{p.sample_freq} {p.sample_wait}  sample_freq, sample_wait",
+          f"{_num(p.error_tol)}  error_tol",
+          f"{p.update_scheme}       Update scheme (0 = Euler, 1=1S)",
+          f"{p.do_CL}        do_CL",
+          "",
+          "# Film parameters",
+          f"{p.do_film}        do_film flag, creates walls in dim-1 direction",
+          f"{_num(p.wallT)} {_num(p.wallXi)}  Wall thickness, interfacial width",
+          f"{_num(p.top_wall_lamA)} {_num(p.top_wall_lamB)}  Top wall interaction parameters, lambda_A, lambda_B",
+          f"{_num(p.bot_wall_lamA)} {_num(p.bot_wall_lamB)}  Bottom wall interaction parameters, lambda_A, lambda_B",
+          "",
+          "# Channel wall parameters",
+          f"{p.do_channel} {p.hor_dir} {p.vert_dir}          do_channel flag",
+          f"{_num(p.wall_width)} {_num(p.channel_width)} {_num(p.channel_xi)}    wall_width channel_width xi",
+          "",
+          "# Nanoparticle parameters",
+          f"{p.n_exp_nr}       n_exp_nr, number of explicit nanoparticles",
+          f"{p.do_fld_np}                do_fld_np, 1 for field-based particles, 0 for no",
+          f"{p.np_type}                np_type, 1=sphere, 2=nanorod",
+          f"{p.np_chem}                np_chem, 0=neutral, 1=A",
+          f"{_num(p.sigma)}          sigma",
+          f"{p.Ng}             Ng",
+          f"{p.Nu}               Nu, number of points per pi for spherical integration",
+          f"{_num(p.np_frac)}              np_frac, volume fraction of field-based nps",
+          f"{_num(p.L_nr)} {_num(p.R_nr)} {_num(p.xi_nr)}      L_nr, R_nr, xi_nr for all nps"]
+    for j in range(min(p.n_exp_nr, 2)):
+        L.append(f"{vec(p.exp_nr_c[j])}   Nanorod center x, y, z for np {j + 1}")
+        L.append(f"{vec(p.exp_nr_u[j])}  Nanorod orientation vector x, y, z for np {j + 1}")
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
+def write_res(path: str, a, p: ScftParams) -> None:
+    """Writes a start field as a .res file for read_one_resume_file
+    (code/io_utils.cpp:137-177): one point per line, the `dim` coordinates
+    n dx then Re Im, x fastest, a blank line after each x row in 2D."""
+    a = np.asarray(a, dtype=np.complex128)
+    nx, ny, nz = p.grid()
+    dx = [p.L[k] / p.Nx[k] for k in range(p.dim)]
+    out = []
+    for i in range(p.M):
+        n = (i % nx, (i // nx) % ny, i // (nx * ny))
+        c = " ".join(repr(float(n[k] * dx[k])) for k in range(p.dim))
+        out.append(f"{c} {float(a[i].real)!r} {float(a[i].imag)!r}")
+        if p.dim == 2 and n[0] == nx - 1:
+            out.append("")
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def read_bin(path: str):
+    """Reads a <name>.p0.bin dump (code/io_utils.cpp:375-397): returns
+    (dim, Nx, L, nprocs, data)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    dim = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    off = 4
+    nx = np.frombuffer(raw, np.int32, dim, off).tolist()
+    off += 4 * dim
+    L = np.frombuffer(raw, np.float64, dim, off).tolist()
+    off += 8 * dim
+    nprocs = int(np.frombuffer(raw, np.int32, 1, off)[0])
+    ml = int(np.frombuffer(raw, np.int32, 1, off + 4)[0])
+    data = np.frombuffer(raw, np.complex128, ml, off + 8).copy()
+    return dim, nx, L, nprocs, data
+
+
+def read_data_dat(path: str) -> np.ndarray:
+    """data.dat rows (code/simulate.cpp:132-135): iter, Re H, Im H,
+    Re -log Qd, Im -log Qd, -log Qha (2), -log Qp + smwp_min (2), error."""
+    rows = [list(map(float, l.split())) for l in open(path) if l.strip()]
+    return np.array(rows, dtype=np.float64).reshape(-1, 10)
+
+
+# --------------------------------------------------------------------------
 # library binding
 # --------------------------------------------------------------------------
 _lib = None
@@ -295,6 +413,7 @@ def load_library(path: str | None = None):
         "scft_get_Q": (ctypes.c_int, [vp, ctypes.c_int, dp]),
         "scft_fft": (ctypes.c_int, [vp, ctypes.c_int, dp, ctypes.c_int]),
         "scft_get_noise": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp]),
+        "scft_get_averages": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp]),
         "scft_set_plates": (ctypes.c_int, [vp, ctypes.c_int, dp, dp]),
         "scft_grad_plates": (ctypes.c_int, [vp, ctypes.c_int, dp, dp]),
         "scft_calc_poly_density": (ctypes.c_int, [vp]),
@@ -437,6 +556,14 @@ class Context:
         out = [np.empty(self.M(sys), np.complex128) for _ in range(3)]
         self._check(self.lib.scft_get_noise(self.h, sys, *[_dptr(x) for x in out]), "scft_get_noise")
         return out
+
+    def get_averages(self, sys=0):
+        """(avg rhoA, avg rhoB, n_samples) of the complex-Langevin run."""
+        a = np.empty(self.M(sys), np.complex128)
+        b = np.empty(self.M(sys), np.complex128)
+        n = np.zeros(1)
+        self._check(self.lib.scft_get_averages(self.h, sys, _dptr(a), _dptr(b), _dptr(n)), "scft_get_averages")
+        return a, b, float(n[0])
 
     def fft(self, a, direction: int, sys=0):
         x = np.array(a, dtype=np.complex128, copy=True)

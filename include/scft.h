@@ -161,6 +161,17 @@ int scft_grad_plates(scft_ctx* ctx, int sys, const double* axis, double* out);
 int scft_get_noise(scft_ctx* ctx, int sys, double* eta_wpl, double* eta_abp,
                    double* eta_abm);
 
+/* Complex-Langevin density averages, accumulate_all_averages
+ * (code/array_utils.cpp:13-35) as sampled by the main loop: scft_iterate adds
+ * rhoA and rhoB after the update of every iteration it with
+ * it >= sample_wait and it % sample_freq == 0 (code/simulate.cpp:91-92),
+ * it counting updates from 0 at create.  Returns the sums divided by the
+ * sample count as write_avg_data_bin does (code/io_utils.cpp:368-369; sums,
+ * i.e. zeros, while the count is 0) and the count in *n_samples.  Any output
+ * pointer may be NULL.  SCFT_EINVAL if the system has do_CL = 0. */
+int scft_get_averages(scft_ctx* ctx, int sys, double* avg_rhoA, double* avg_rhoB,
+                      double* n_samples);
+
 /* FFT seam, fft_fwd_wrapper / fft_bck_wrapper (code/fft_wrappers.cpp:4-67):
  * dir = -1 forward, scaled by 1/M; dir = +1 backward, unscaled.  `inout` is a
  * host array of M interleaved complex values of system `sys`'s grid. */
@@ -180,7 +191,8 @@ int scft_update_euler(scft_ctx* ctx);
 int scft_calc_H(scft_ctx* ctx, double* H_re_im);
 
 /* The loop body of simulate() (code/simulate.cpp:85-157) for n_iter
- * iterations of the update selected by update_scheme, without H or output.
+ * iterations of the update selected by update_scheme, with the
+ * complex-Langevin averages (scft_get_averages) but without H or output.
  * Captured into a HIP graph after the first call. */
 int scft_iterate(scft_ctx* ctx, int n_iter);
 

@@ -971,10 +971,12 @@ int oracle_simulate(void* h, int max_rows, double* rows, const char* data_path) 
       }
       nrows++;
       if (otp) {
-        cd one = -std::log(cd(1.0));
+        // Qha = Qp = 1 and smwp_min = 0 here (code/calc_poly_density.cpp:122,151)
+        cd lqha = -std::log(cd(1.0));
+        cd lqp = -std::log(cd(1.0)) + cd(0.0);
         fprintf(otp, "%d %5.6lf %1.3e %5.6lf %1.3e %5.6lf %1.3e %5.6lf %1.3e %1.3e\n",
-                o->iter, H.real(), H.imag(), mlq.real(), mlq.imag(), one.real(),
-                one.imag(), one.real(), one.imag(), error);
+                o->iter, H.real(), H.imag(), mlq.real(), mlq.imag(), lqha.real(),
+                lqha.imag(), lqp.real(), lqp.imag(), error);
       }
     }
     if (!o->p.do_CL && o->iter > 25 && error < o->p.error_tol && close > 10) break;

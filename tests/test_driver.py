@@ -1,0 +1,241 @@
+"""The host driver scft_run (simulate() loop, code/simulate.cpp:16-167) and the
+reference file formats it reads and writes: bcp.input, .res start fields,
+data.dat rows and .p0.bin dumps.
+
+CPU tests cover the format helpers and the driver's failure paths; the GPU
+tests run the driver against the oracle's simulate() on the same bcp.input
+and .res files.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle
+import scft_amd
+from scft_amd import Workload, seeded_fields
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(HERE, "..", "aligned-nanoplate-separation-and-rotation_amd", "scft_run")
+GOLDEN = os.path.join(HERE, "golden")
+
+SCALARS = ["dim", "Nda", "Ndb", "phiH", "Nah", "a_smear", "rho0", "chi_ab", "chi_bc", "chi_ac", "kappa",
+           "lam_imag", "lam_real", "itermax", "print_freq", "sample_freq", "sample_wait", "error_tol",
+           "update_scheme", "do_CL", "do_film", "wallT", "wallXi", "top_wall_lamA", "top_wall_lamB",
+           "bot_wall_lamA", "bot_wall_lamB", "do_channel", "hor_dir", "vert_dir", "wall_width",
+           "channel_width", "channel_xi", "n_exp_nr", "do_fld_np", "np_type", "np_chem", "sigma", "Ng", "Nu",
+           "np_frac", "L_nr", "R_nr", "xi_nr"]
+
+
+def same_params(a, b):
+    for k in SCALARS:
+        assert getattr(a, k) == getattr(b, k), k
+    for k in ("ic_flag", "ic_pre", "ic_dir", "ic_period"):
+        assert list(getattr(a, k)) == list(getattr(b, k)), k
+    for d in range(a.dim):
+        assert a.Nx[d] == b.Nx[d] and a.L[d] == b.L[d]
+    for j in range(min(a.n_exp_nr, 2)):
+        for d in range(a.dim):
+            assert a.exp_nr_c[j][d] == b.exp_nr_c[j][d]
+            assert a.exp_nr_u[j][d] == b.exp_nr_u[j][d]
+
+
+def need_driver():
+    if not os.path.exists(DRIVER):
+        pytest.skip("scft_run is not built (python aligned-nanoplate-separation-and-rotation_amd/build.py)")
+
+
+# ------------------------------------------------------------------ formats
+@pytest.mark.parametrize("w", [Workload(grid=(16, 16, 32), box_rg=(2.0, 2.0, 4.0), N=16, theta_deg=30.0),
+                               Workload(grid=(24, 20), box_rg=(3.0, 2.5), N=20, np_type=scft_amd.NP_ROD, theta_deg=90.0)])
+def test_write_input_round_trip(tmp_path, w):
+    p = w.params()
+    p.ic_flag[0], p.ic_pre[0], p.ic_dir[0], p.ic_period[0] = -2, -1.0, 2, 4.0
+    path = str(tmp_path / "bcp.input")
+    scft_amd.write_input(p, path)
+    q = scft_amd.read_input(path, p.dim)
+    r = pyoracle.read_input(path, p.dim)
+    same_params(q, r)
+    # the reader normalises u; the workload's u is already a unit vector
+    for k in SCALARS:
+        assert getattr(q, k) == pytest.approx(getattr(p, k), rel=1e-15, abs=0), k
+
+
+def test_write_input_reproduces_published_input(tmp_path):
+    src = os.path.join(GOLDEN, "bcp_dist-0.300.input")
+    p = scft_amd.read_input(src, 2)
+    out = str(tmp_path / "bcp.input")
+    scft_amd.write_input(p, out)
+    same_params(scft_amd.read_input(out, 2), p)
+
+
+def test_res_and_bin_formats(tmp_path):
+    p = Workload(grid=(6, 5), box_rg=(2.0, 2.0), N=8).params()
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(p.M) + 1j * rng.standard_normal(p.M)
+    path = str(tmp_path / "wpl.res")
+    scft_amd.write_res(path, a, p)
+    lines = open(path).read().split("\n")
+    # 2D: a blank line after each x row (code/io_utils.cpp:150-155)
+    assert lines[6] == "" and len(lines[0].split()) == 4
+    vals = np.array([list(map(float, l.split())) for l in lines if l.strip()])
+    assert np.array_equal(vals[:, 2] + 1j * vals[:, 3], a)
+    assert vals[7, 0] == pytest.approx(p.L[0] / p.Nx[0]) and vals[7, 1] == pytest.approx(p.L[1] / p.Nx[1])
+    # .p0.bin layout of write_data_bin (code/io_utils.cpp:375-397)
+    b = str(tmp_path / "x.p0.bin")
+    with open(b, "wb") as f:
+        np.array([2, 6, 5], np.int32).tofile(f)
+        np.array([p.L[0], p.L[1]], np.float64).tofile(f)
+        np.array([1, p.M], np.int32).tofile(f)
+        a.astype(np.complex128).tofile(f)
+    dim, nx, L, nprocs, data = scft_amd.read_bin(b)
+    assert (dim, nx, nprocs) == (2, [6, 5], 1) and L == [p.L[0], p.L[1]]
+    assert np.array_equal(data, a)
+
+
+# ------------------------------------------------------------ failure paths
+def test_driver_without_input_fails_cleanly(tmp_path):
+    need_driver()
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "Failed to open bcp.input!" in r.stdout
+    assert "scft_read_input failed" in r.stderr
+
+
+def test_driver_rejects_bad_arguments(tmp_path):
+    need_driver()
+    r = subprocess.run([DRIVER, "--bogus"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_driver_without_gpu_fails_cleanly(tmp_path):
+    need_driver()
+    p = Workload(grid=(16, 16, 32), box_rg=(2.0, 2.0, 4.0), N=16).params()
+    scft_amd.write_input(p, str(tmp_path / "bcp.input"))
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "scft_create failed" in r.stderr
+
+
+# -------------------------------------------------- driver vs oracle (GPU)
+def _prepare(tmp_path, p, seed=5):
+    f = seeded_fields(p, seed, period_rg=2.0, amp=1.0, sigma=0.01)
+    scft_amd.write_input(p, str(tmp_path / "bcp.input"))
+    for name, a in zip(("wpl", "wabp", "wabm"), f):
+        scft_amd.write_res(str(tmp_path / f"{name}.res"), a, p)
+    return f
+
+
+def _run_driver(tmp_path, *args):
+    print("scft_run", tmp_path, args, flush=True)
+    r = subprocess.run([DRIVER, "--dim", "3", *args], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r.stdout
+
+
+def _oracle_rows(tmp_path, p, f):
+    ref = pyoracle.Oracle(scft_amd.read_input(str(tmp_path / "bcp.input"), 3))
+    ref.set_fields(*f)
+    path = str(tmp_path / "oracle_data.dat")
+    ref.simulate(data_path=path)
+    return ref, scft_amd.read_data_dat(path)
+
+
+def _compare_rows(got, want):
+    assert got.shape == want.shape, (got[:, 0], want[:, 0])
+    assert np.array_equal(got[:, 0], want[:, 0])
+    scale = np.max(np.abs(want[:, 1]))
+    # %5.6lf columns: H and -log Qd agree to the print precision
+    assert np.max(np.abs(got[:, 1] - want[:, 1])) <= 2e-6 + 1e-10 * scale
+    assert np.max(np.abs(got[:, 3] - want[:, 3])) <= 2e-6
+    # imaginary parts are round-off of real fields
+    assert np.max(np.abs(got[:, [2, 4]])) < 1e-9 * scale
+    assert np.array_equal(got[:, 5:9], want[:, 5:9])
+    # error: 4 printed digits
+    assert np.all(np.abs(got[:, 9] - want[:, 9]) <= 1e-3 * want[:, 9] + 1e-12)
+
+
+@pytest.mark.gpu
+def test_driver_matches_oracle_full_run(tmp_path):
+    need_driver()
+    w = Workload(grid=(16, 16, 32), box_rg=(2.0, 2.0, 4.0), N=16, itermax=37, print_freq=5, error_tol=1e-14)
+    p = w.params()
+    f = _prepare(tmp_path, p)
+    out = _run_driver(tmp_path)
+    assert "Read wpl.res" in out and "---Entering main loop---" in out
+    got = scft_amd.read_data_dat(str(tmp_path / "data.dat"))
+    ref, want = _oracle_rows(tmp_path, p, f)
+    assert list(want[:, 0]) == [0, 5, 10, 15, 20, 25, 30, 35]
+    _compare_rows(got, want)
+    # last dumps: fields after iteration 35, in the reference binary layout
+    dim, nx, L, nprocs, wpl = scft_amd.read_bin(str(tmp_path / "wpl.p0.bin"))
+    assert (dim, nx, nprocs) == (3, [16, 16, 32], 1) and L == [p.L[0], p.L[1], p.L[2]]
+    for name in ("rhoda", "rhodb", "rhoda_c", "rhodb_c", "wabp", "wabm"):
+        assert os.path.exists(tmp_path / f"{name}.p0.bin"), name
+    # the run ends at itermax = 37, two iterations after the last dump
+    ref2 = pyoracle.Oracle(scft_amd.read_input(str(tmp_path / "bcp.input"), 3))
+    ref2.set_fields(*f)
+    ref2.calc_poly_density()
+    for _ in range(36):
+        ref2.update_1s()
+    assert np.max(np.abs(wpl - ref2.array("wpl"))) < 1e-9 * np.max(np.abs(ref2.array("wpl")))
+    rho = scft_amd.read_bin(str(tmp_path / "rhoda_c.p0.bin"))[4]
+    assert np.max(np.abs(rho - ref2.array("rhoda"))) < 1e-9 * np.max(np.abs(ref2.array("rhoda")))
+
+
+@pytest.mark.gpu
+def test_driver_stop_rule_matches_oracle(tmp_path):
+    # stop when !do_CL && iter > 25 && error < error_tol && close > 10
+    # (code/simulate.cpp:149); error_tol is set from the oracle's own error
+    # column so that the rule fires between iterations 26 and 60
+    need_driver()
+    w = Workload(grid=(16, 16, 32), box_rg=(2.0, 2.0, 4.0), N=16, itermax=200, print_freq=1, error_tol=1e-14)
+    p = w.params()
+    f = _prepare(tmp_path, p)
+    _, probe = _oracle_rows(tmp_path, p, f)
+    err = probe[:, 9]
+    p.error_tol = float(np.sqrt(err[40] * err[41]))  # between two printed rows
+    scft_amd.write_input(p, str(tmp_path / "bcp.input"))
+    out = _run_driver(tmp_path, "--no-bin")
+    assert "Tolerance reached" in out
+    got = scft_amd.read_data_dat(str(tmp_path / "data.dat"))
+    _, want = _oracle_rows(tmp_path, p, f)
+    assert 26 <= want[-1, 0] < 200
+    _compare_rows(got, want)
+    assert not os.path.exists(tmp_path / "wpl.p0.bin")
+
+
+@pytest.mark.gpu
+def test_driver_cl_run_writes_averages(tmp_path):
+    need_driver()
+    w = Workload(grid=(16, 16, 32), box_rg=(2.0, 2.0, 4.0), N=16, C=5.0, update_scheme=0, do_CL=1, dt=0.001,
+                 itermax=12, print_freq=4)
+    p = w.params()
+    p.sample_freq, p.sample_wait = 2, 3
+    f = _prepare(tmp_path, p)
+    env = dict(os.environ, SCFT_NO_TUNE="1")
+    print("scft_run CL", tmp_path, flush=True)
+    r = subprocess.run([DRIVER, "--seed", "99"], cwd=tmp_path, capture_output=True, text=True, timeout=300,
+                       env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert " + i" in r.stdout                # CL runs print Im H
+    rows = scft_amd.read_data_dat(str(tmp_path / "data.dat"))
+    assert list(rows[:, 0]) == [0, 4, 8, 12]
+    avg = scft_amd.read_bin(str(tmp_path / "avg_rhoda.p0.bin"))[4]
+    # same run in this process: averages of iterations 4, 6, 8, 10, 12
+    os.environ["SCFT_NO_TUNE"] = "1"
+    try:
+        q = scft_amd.read_input(str(tmp_path / "bcp.input"), 3)
+        q.seed = 99
+        ctx = scft_amd.Context(q)
+    finally:
+        del os.environ["SCFT_NO_TUNE"]
+    ctx.set_fields(*f)
+    ctx.calc_poly_density()
+    ctx.iterate(13)
+    a, _, n = ctx.get_averages()
+    assert n == 5.0
+    assert np.max(np.abs(avg - a)) <= 1e-12 * np.max(np.abs(a))
+    ctx.close()

@@ -442,6 +442,41 @@ def test_cl_fp32_tracks_fp64_and_replicas():
         c.close()
 
 
+def test_cl_averages_follow_sampling_rule():
+    # accumulate_all_averages (code/array_utils.cpp:13-35) after the update of
+    # iteration it when it >= sample_wait and it % sample_freq == 0
+    # (code/simulate.cpp:91-92); the dump divides by n_samples
+    # (code/io_utils.cpp:368-369)
+    say("cl averages")
+    p = _cl_workload().params()
+    p.sample_freq, p.sample_wait = 3, 4
+    ctx = scft_amd.Context(p)
+    ctx.calc_poly_density()
+    a0, b0, n0 = ctx.get_averages()
+    assert n0 == 0.0 and not np.any(a0) and not np.any(b0)
+    sa = np.zeros(p.M, np.complex128)
+    sb = np.zeros(p.M, np.complex128)
+    n = 0
+    for it in range(16):
+        ctx.iterate(1)
+        if it >= p.sample_wait and it % p.sample_freq == 0:
+            ra, rb = ctx.get_density()
+            sa += ra
+            sb += rb
+            n += 1
+    a, b, ns = ctx.get_averages()
+    assert ns == n == 4                       # it = 6, 9, 12, 15
+    assert np.array_equal(a, sa / n) and np.array_equal(b, sb / n)
+    # the averages are over distinct noisy states
+    assert not np.allclose(a, ctx.get_density()[0])
+    ctx.close()
+    q = _cl_workload(do_CL=0, update_scheme=1).params()
+    plain = scft_amd.Context(q)
+    with pytest.raises(scft_amd.ScftError):
+        plain.get_averages()
+    plain.close()
+
+
 # ------------------------------------------------------ trajectories (golden)
 def _golden(name):
     here = os.path.join(os.path.dirname(__file__), "golden", name)
