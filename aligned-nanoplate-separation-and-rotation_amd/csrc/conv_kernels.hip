@@ -103,6 +103,74 @@ __global__ __launch_bounds__((PlaneCfg<NX, NY, GX, GY>::T)) void k_plane_op(
   }
 }
 
+// The same plane operation without the staging copies: x lines are loaded
+// from global memory straight into the lane groups, their results go to LDS
+// once, and the y lines are stored from registers to global memory with the
+// multiplier and scale applied (lanes of consecutive groups hold consecutive
+// x, so each store covers whole 128-byte rows).  In place (in == out) is
+// safe: a block reads its whole plane before the barrier.
+template <typename R, int MODE, int NX, int NY, int GX, int GY>
+__global__ __launch_bounds__((PlaneCfg<NX, NY, GX, GY>::T)) void k_plane_direct(
+    const PassDesc* __restrict__ descs, int step) {
+  static_assert(NY > 1, "direct plane pass needs a y axis");
+  constexpr int T = PlaneCfg<NX, NY, GX, GY>::T, PX = plane_px<R, NX>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* twx = lds + NY * PX;
+  cpx<R>* twy = twx + NX;
+  R* gx = reinterpret_cast<R*>(twy + NY);
+  R* gy = gx + NX;
+  const PassDesc& d = descs[blockIdx.y];
+  if (d.steps_active && step >= d.steps_active) return;
+  const int z = blockIdx.x;
+  if (z >= d.nz) return;
+  const int tid = threadIdx.x;
+  const long poff = (long)z * (NX * NY);
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step + d.in_soff) * d.in_step + poff;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step + d.out_soff) * d.out_step + poff;
+  if (tid < NX) {
+    twx[tid] = reinterpret_cast<const cpx<R>*>(d.tw0)[tid];
+    if (MODE == 0) gx[tid] = reinterpret_cast<const R*>(d.g0)[tid];
+  }
+  if (tid < NY) {
+    twy[tid] = reinterpret_cast<const cpx<R>*>(d.tw1)[tid];
+    if (MODE == 0) gy[tid] = reinterpret_cast<const R*>(d.g1)[tid];
+  }
+  __syncthreads();
+  {
+    const int grp = tid / GX, t = tid - grp * GX;
+    for (int row = grp; row < NY; row += T / GX) {
+      const cpx<R>* src = in + row * NX;
+      cpx<R>* dst = lds + row * PX;
+      line_op_io<MODE, R, NX, GX>(
+          t, twx, gx, [src](int i) { return src[i]; }, [dst](int i, cpx<R> v) { dst[i] = v; });
+    }
+  }
+  __syncthreads();
+  const cpx<R>* post = nullptr;
+  R scale = R(1);
+  if (d.last) {
+    const void* pp = (step < d.post_switch) ? d.post_a : d.post_b;
+    if (pp) post = reinterpret_cast<const cpx<R>*>(pp) + poff;
+    scale = R(d.scale);
+  }
+  {
+    const int grp = tid / GY, t = tid - grp * GY;
+    for (int col = grp; col < NX; col += T / GY) {
+      const cpx<R>* src = lds + col;
+      cpx<R>* dst = out + col;
+      const cpx<R>* pc = post ? post + col : nullptr;
+      line_op_io<MODE, R, NY, GY>(
+          t, twy, gy, [src](int i) { return src[i * PX]; },
+          [dst, pc, scale](int i, cpx<R> v) {
+            if (pc) v = cmul(v, pc[i * NX]);
+            if (scale != R(1)) v = cscale(v, scale);
+            dst[i * NX] = v;
+          });
+    }
+  }
+}
+
 template <int NZ, int G, int TL>
 struct ZCfg {
   static constexpr int T = TL * G;
@@ -174,32 +242,43 @@ __global__ __launch_bounds__((ZCfg<NZ, G, TL>::T)) void k_z_op(const PassDesc* _
 // ---------------------------------------------------------------------------
 // variants: (NX, NY, GX, GY) for planes, (NZ, G, TL) for z tiles
 // ---------------------------------------------------------------------------
+// plane variant (NX, NY, GX, GY, D): D = 1 for k_plane_direct
 #define SCFT_PLANE_VARIANTS(X)                                                             \
-  X(8, 8, 4, 4) X(16, 16, 4, 4) X(24, 16, 4, 4) X(32, 32, 4, 4) X(32, 32, 8, 8)            \
-  X(56, 64, 4, 4) X(56, 64, 8, 8) X(64, 64, 4, 4) X(64, 64, 8, 8) X(64, 64, 16, 16)        \
-  X(72, 64, 4, 4) X(72, 64, 8, 8) X(96, 96, 4, 4) X(96, 96, 8, 8)
+  X(8, 8, 4, 4, 0) X(16, 16, 4, 4, 0) X(24, 16, 4, 4, 0) X(32, 32, 4, 4, 0)                \
+  X(32, 32, 8, 8, 0) X(56, 64, 4, 4, 0) X(56, 64, 8, 8, 0) X(64, 64, 4, 4, 0)              \
+  X(64, 64, 8, 8, 0) X(64, 64, 16, 16, 0) X(72, 64, 4, 4, 0) X(72, 64, 8, 8, 0)            \
+  X(96, 96, 4, 4, 0) X(96, 96, 8, 8, 0)                                                    \
+  X(32, 32, 4, 4, 1) X(32, 32, 8, 8, 1) X(56, 64, 4, 4, 1) X(56, 64, 8, 8, 1)              \
+  X(64, 64, 4, 4, 1) X(64, 64, 8, 8, 1) X(72, 64, 4, 4, 1) X(72, 64, 8, 8, 1)              \
+  X(96, 96, 4, 4, 1) X(96, 96, 8, 8, 1)
 #define SCFT_Z_VARIANTS(X)                                                                 \
   X(16, 4, 32) X(32, 4, 32) X(32, 8, 32) X(64, 8, 32) X(64, 8, 16) X(128, 8, 32)           \
   X(128, 16, 32) X(128, 8, 16) X(128, 16, 16) X(192, 8, 32) X(192, 16, 32) X(192, 16, 16)
 
-template <typename R, int MODE, int NX, int NY, int GX, int GY>
+template <typename R, int MODE, int NX, int NY, int GX, int GY, int D>
 static hipError_t launch_plane_m(const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
   using C = PlaneCfg<NX, NY, GX, GY>;
   constexpr size_t lds = plane_lds_bytes<R, NX, NY, GX, GY>();
+  const void* fn = D ? (const void*)k_plane_direct<R, MODE, NX, NY, GX, GY>
+                     : (const void*)k_plane_op<R, MODE, NX, NY, GX, GY>;
   static bool attr = false;
   if (!attr && lds > 64 * 1024)
-    (void)hipFuncSetAttribute((const void*)k_plane_op<R, MODE, NX, NY, GX, GY>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   attr = true;
-  hipLaunchKernelGGL((k_plane_op<R, MODE, NX, NY, GX, GY>), dim3(gridx, njobs), dim3(C::T), lds, st, d, step);
+  if (D)
+    hipLaunchKernelGGL((k_plane_direct<R, MODE, NX, NY, GX, GY>), dim3(gridx, njobs), dim3(C::T), lds, st, d,
+                       step);
+  else
+    hipLaunchKernelGGL((k_plane_op<R, MODE, NX, NY, GX, GY>), dim3(gridx, njobs), dim3(C::T), lds, st, d,
+                       step);
   return hipGetLastError();
 }
 
-template <typename R, int NX, int NY, int GX, int GY>
+template <typename R, int NX, int NY, int GX, int GY, int D>
 static hipError_t launch_plane_t(int mode, const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
-  if (mode == MODE_CONV) return launch_plane_m<R, MODE_CONV, NX, NY, GX, GY>(d, njobs, gridx, step, st);
-  if (mode == MODE_FWD) return launch_plane_m<R, MODE_FWD, NX, NY, GX, GY>(d, njobs, gridx, step, st);
-  return launch_plane_m<R, MODE_INV, NX, NY, GX, GY>(d, njobs, gridx, step, st);
+  if (mode == MODE_CONV) return launch_plane_m<R, MODE_CONV, NX, NY, GX, GY, D>(d, njobs, gridx, step, st);
+  if (mode == MODE_FWD) return launch_plane_m<R, MODE_FWD, NX, NY, GX, GY, D>(d, njobs, gridx, step, st);
+  return launch_plane_m<R, MODE_INV, NX, NY, GX, GY, D>(d, njobs, gridx, step, st);
 }
 
 template <typename R, int MODE, int NZ, int G, int TL>
@@ -224,7 +303,7 @@ static hipError_t launch_z_t(int mode, const PassDesc* d, int njobs, int gridx, 
 
 int fast_plane_variants(int prec, int nx, int ny, int* ids, int max_ids) {
   int n = 0, id = 0;
-#define X(a, b, ga, gb)                                                                             \
+#define X(a, b, ga, gb, dd)                                                                         \
   if (nx == a && ny == b && n < max_ids &&                                                          \
       (prec == 0 ? plane_lds_bytes<double, a, b, ga, gb>() : plane_lds_bytes<float, a, b, ga, gb>()) <= \
           160 * 1024)                                                                               \
@@ -252,10 +331,10 @@ int fast_z_variants(int prec, int nz, int* ids, int* tiles, int max_ids) {
 hipError_t launch_fast_plane(int prec, int mode, int variant, const PassDesc* d, int njobs, int gridx,
                              int step, hipStream_t st) {
   int id = 0;
-#define X(a, b, ga, gb)                                                                        \
-  if (variant == id)                                                                           \
-    return prec == 0 ? launch_plane_t<double, a, b, ga, gb>(mode, d, njobs, gridx, step, st)   \
-                     : launch_plane_t<float, a, b, ga, gb>(mode, d, njobs, gridx, step, st);   \
+#define X(a, b, ga, gb, dd)                                                                        \
+  if (variant == id)                                                                               \
+    return prec == 0 ? launch_plane_t<double, a, b, ga, gb, dd>(mode, d, njobs, gridx, step, st)   \
+                     : launch_plane_t<float, a, b, ga, gb, dd>(mode, d, njobs, gridx, step, st);   \
   id++;
   SCFT_PLANE_VARIANTS(X)
 #undef X

@@ -283,34 +283,43 @@ __device__ __forceinline__ void group_conv(cpx<R>* v, int t, const cpx<R>* __res
 }
 
 // One line transform in the lane-group layout: MODE 0 convolution,
-// 1 forward DFT, 2 backward DFT.  Reads / writes go through `at(i)`, the
-// address of point i of the line.  Forward output and backward input are
-// in natural frequency order.
-template <int MODE, typename R, int NL, int G, typename At>
-__device__ __forceinline__ void line_op(int t, const cpx<R>* __restrict__ tw, const R* __restrict__ g, At at) {
+// 1 forward DFT, 2 backward DFT.  Point i of the line is read with ld(i)
+// and written with st(i, value); forward output and backward input are in
+// natural frequency order.
+template <int MODE, typename R, int NL, int G, typename Ld, typename St>
+__device__ __forceinline__ void line_op_io(int t, const cpx<R>* __restrict__ tw, const R* __restrict__ g,
+                                           Ld ld, St st) {
   constexpr int E = NL / G;
   cpx<R> v[E];
   if (MODE == 2) {
     const int kb = E * bitrev_g<G>(t);
 #pragma unroll
-    for (int k = 0; k < E; k++) v[k] = *at(kb + k);
+    for (int k = 0; k < E; k++) v[k] = ld(kb + k);
     group_inv<R, NL, G>(v, t, tw);
 #pragma unroll
-    for (int m = 0; m < E; m++) *at(t + G * m) = v[m];
+    for (int m = 0; m < E; m++) st(t + G * m, v[m]);
   } else {
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = *at(t + G * m);
+    for (int m = 0; m < E; m++) v[m] = ld(t + G * m);
     if (MODE == 0) {
       group_conv<R, NL, G>(v, t, tw, g);
 #pragma unroll
-      for (int m = 0; m < E; m++) *at(t + G * m) = v[m];
+      for (int m = 0; m < E; m++) st(t + G * m, v[m]);
     } else {
       group_fwd<R, NL, G>(v, t, tw);
       const int kb = E * bitrev_g<G>(t);
 #pragma unroll
-      for (int k = 0; k < E; k++) *at(kb + k) = v[k];
+      for (int k = 0; k < E; k++) st(kb + k, v[k]);
     }
   }
+}
+
+// line_op_io reading and writing in place through `at(i)`, the address of
+// point i of the line.
+template <int MODE, typename R, int NL, int G, typename At>
+__device__ __forceinline__ void line_op(int t, const cpx<R>* __restrict__ tw, const R* __restrict__ g, At at) {
+  line_op_io<MODE, R, NL, G>(
+      t, tw, g, [&](int i) { return *at(i); }, [&](int i, cpx<R> v) { *at(i) = v; });
 }
 
 }  // namespace scft
