@@ -137,13 +137,17 @@ __global__ __launch_bounds__((PlaneCfg<NX, NY, GX, GY>::T)) void k_plane_direct(
     if (MODE == 0) gy[tid] = reinterpret_cast<const R*>(d.g1)[tid];
   }
   __syncthreads();
+  constexpr int SZ = (int)sizeof(cpx<R>);
+  constexpr unsigned PB = (unsigned)(NX * NY * SZ);
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(in, PB);
   {
     const int grp = tid / GX, t = tid - grp * GX;
     for (int row = grp; row < NY; row += T / GX) {
-      const cpx<R>* src = in + row * NX;
       cpx<R>* dst = lds + row * PX;
+      const int vo = row * NX * SZ;
       line_op_io<MODE, R, NX, GX>(
-          t, twx, gx, [src](int i) { return src[i]; }, [dst](int i, cpx<R> v) { dst[i] = v; });
+          t, twx, gx, [rin, vo](int b, int o) { return buf_ld<R>(rin, vo + b * SZ, o * SZ); },
+          [dst](int b, int o, cpx<R> v) { dst[b + o] = v; });
     }
   }
   __syncthreads();
@@ -154,18 +158,19 @@ __global__ __launch_bounds__((PlaneCfg<NX, NY, GX, GY>::T)) void k_plane_direct(
     if (pp) post = reinterpret_cast<const cpx<R>*>(pp) + poff;
     scale = R(d.scale);
   }
+  const __amdgpu_buffer_rsrc_t rout = buf_rsrc(out, PB);
+  const __amdgpu_buffer_rsrc_t rpost = buf_rsrc(post ? post : out, PB);
   {
     const int grp = tid / GY, t = tid - grp * GY;
     for (int col = grp; col < NX; col += T / GY) {
       const cpx<R>* src = lds + col;
-      cpx<R>* dst = out + col;
-      const cpx<R>* pc = post ? post + col : nullptr;
       line_op_io<MODE, R, NY, GY>(
-          t, twy, gy, [src](int i) { return src[i * PX]; },
-          [dst, pc, scale](int i, cpx<R> v) {
-            if (pc) v = cmul(v, pc[i * NX]);
+          t, twy, gy, [src](int b, int o) { return src[(b + o) * PX]; },
+          [rout, rpost, post, scale, col](int b, int o, cpx<R> v) {
+            const int vo = (col + b * NX) * SZ, so = o * NX * SZ;
+            if (post) v = cmul(v, buf_ld<R>(rpost, vo, so));
             if (scale != R(1)) v = cscale(v, scale);
-            dst[i * NX] = v;
+            buf_st(v, rout, vo, so);
           });
     }
   }
@@ -239,6 +244,58 @@ __global__ __launch_bounds__((ZCfg<NZ, G, TL>::T)) void k_z_op(const PassDesc* _
   }
 }
 
+// z pencils without LDS staging: lane group g of a block holds line l0 + g,
+// loaded from and stored to global memory directly.  With G lanes per line
+// a wavefront holds 64 / G consecutive lines, so for G <= 8 every access
+// instruction covers runs of 64 / G consecutive points (128 bytes at G = 4
+// in FP32).  Only the twiddle and multiplier tables sit in LDS.
+template <typename R, int MODE, int NZ, int G, int TL>
+__global__ __launch_bounds__((ZCfg<NZ, G, TL>::T)) void k_z_direct(const PassDesc* __restrict__ descs,
+                                                                   int step) {
+  using C = ZCfg<NZ, G, TL>;
+  constexpr int T = C::T;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* tw = reinterpret_cast<cpx<R>*>(smem_raw);
+  R* g = reinterpret_cast<R*>(tw + NZ);
+  const PassDesc& d = descs[blockIdx.y];
+  if (d.steps_active && step >= d.steps_active) return;
+  const int tid = threadIdx.x;
+  const int nlines = d.nx * d.ny;
+  const long S = nlines;
+  const int l0 = blockIdx.x * TL;
+  if (l0 >= nlines) return;
+  for (int i = tid; i < NZ; i += T) {
+    tw[i] = reinterpret_cast<const cpx<R>*>(d.tw0)[i];
+    if (MODE == 0) g[i] = reinterpret_cast<const R*>(d.g0)[i];
+  }
+  __syncthreads();
+  const int grp = tid / G, t = tid - grp * G;
+  const int l = l0 + grp;
+  if (l >= nlines) return;
+  constexpr int SZ = (int)sizeof(cpx<R>);
+  const unsigned AB = (unsigned)(S * NZ * SZ);
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step + d.in_soff) * d.in_step;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step + d.out_soff) * d.out_step;
+  const cpx<R>* post = nullptr;
+  R scale = R(1);
+  if (d.last) {
+    const void* pp = (step < d.post_switch) ? d.post_a : d.post_b;
+    if (pp) post = reinterpret_cast<const cpx<R>*>(pp);
+    scale = R(d.scale);
+  }
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(in, AB), rout = buf_rsrc(out, AB);
+  const __amdgpu_buffer_rsrc_t rpost = buf_rsrc(post ? post : out, AB);
+  const int SB = (int)S * SZ;  // bytes between z neighbours
+  line_op_io<MODE, R, NZ, G>(
+      t, tw, g, [rin, l, SB](int b, int o) { return buf_ld<R>(rin, l * SZ + b * SB, o * SB); },
+      [rout, rpost, post, scale, l, SB](int b, int o, cpx<R> v) {
+        const int vo = l * SZ + b * SB, so = o * SB;
+        if (post) v = cmul(v, buf_ld<R>(rpost, vo, so));
+        if (scale != R(1)) v = cscale(v, scale);
+        buf_st(v, rout, vo, so);
+      });
+}
+
 // ---------------------------------------------------------------------------
 // variants: (NX, NY, GX, GY) for planes, (NZ, G, TL) for z tiles
 // ---------------------------------------------------------------------------
@@ -251,26 +308,34 @@ __global__ __launch_bounds__((ZCfg<NZ, G, TL>::T)) void k_z_op(const PassDesc* _
   X(32, 32, 4, 4, 1) X(32, 32, 8, 8, 1) X(56, 64, 4, 4, 1) X(56, 64, 8, 8, 1)              \
   X(64, 64, 4, 4, 1) X(64, 64, 8, 8, 1) X(72, 64, 4, 4, 1) X(72, 64, 8, 8, 1)              \
   X(96, 96, 4, 4, 1) X(96, 96, 8, 8, 1)
+// z variant (NZ, G, TL, D): D = 1 for k_z_direct
 #define SCFT_Z_VARIANTS(X)                                                                 \
-  X(16, 4, 32) X(32, 4, 32) X(32, 8, 32) X(64, 8, 32) X(64, 8, 16) X(128, 8, 32)           \
-  X(128, 16, 32) X(128, 8, 16) X(128, 16, 16) X(192, 8, 32) X(192, 16, 32) X(192, 16, 16)
+  X(16, 4, 32, 0) X(32, 4, 32, 0) X(32, 8, 32, 0) X(64, 8, 32, 0) X(64, 8, 16, 0)          \
+  X(128, 8, 32, 0) X(128, 16, 32, 0) X(128, 8, 16, 0) X(128, 16, 16, 0) X(192, 8, 32, 0)   \
+  X(192, 16, 32, 0) X(192, 16, 16, 0)                                                      \
+  X(64, 4, 16, 1) X(64, 4, 64, 1) X(64, 8, 32, 1) X(128, 4, 16, 1) X(128, 4, 64, 1)        \
+  X(128, 8, 16, 1) X(128, 8, 32, 1) X(192, 8, 16, 1) X(192, 8, 32, 1)
 
 template <typename R, int MODE, int NX, int NY, int GX, int GY, int D>
 static hipError_t launch_plane_m(const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
   using C = PlaneCfg<NX, NY, GX, GY>;
   constexpr size_t lds = plane_lds_bytes<R, NX, NY, GX, GY>();
-  const void* fn = D ? (const void*)k_plane_direct<R, MODE, NX, NY, GX, GY>
-                     : (const void*)k_plane_op<R, MODE, NX, NY, GX, GY>;
   static bool attr = false;
-  if (!attr && lds > 64 * 1024)
-    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  attr = true;
-  if (D)
+  if constexpr (D) {
+    if (!attr && lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)k_plane_direct<R, MODE, NX, NY, GX, GY>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
     hipLaunchKernelGGL((k_plane_direct<R, MODE, NX, NY, GX, GY>), dim3(gridx, njobs), dim3(C::T), lds, st, d,
                        step);
-  else
+  } else {
+    if (!attr && lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)k_plane_op<R, MODE, NX, NY, GX, GY>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
     hipLaunchKernelGGL((k_plane_op<R, MODE, NX, NY, GX, GY>), dim3(gridx, njobs), dim3(C::T), lds, st, d,
                        step);
+  }
   return hipGetLastError();
 }
 
@@ -281,30 +346,40 @@ static hipError_t launch_plane_t(int mode, const PassDesc* d, int njobs, int gri
   return launch_plane_m<R, MODE_INV, NX, NY, GX, GY, D>(d, njobs, gridx, step, st);
 }
 
-template <typename R, int MODE, int NZ, int G, int TL>
+template <typename R, int NZ, int G, int TL, int D>
+constexpr size_t z_variant_lds() {
+  return D ? sizeof(cpx<R>) * NZ + sizeof(R) * NZ : z_lds_bytes<R, NZ, G, TL>();
+}
+
+template <typename R, int MODE, int NZ, int G, int TL, int D>
 static hipError_t launch_z_m(const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
   using C = ZCfg<NZ, G, TL>;
-  constexpr size_t lds = z_lds_bytes<R, NZ, G, TL>();
+  constexpr size_t lds = z_variant_lds<R, NZ, G, TL, D>();
   static bool attr = false;
-  if (!attr && lds > 64 * 1024)
-    (void)hipFuncSetAttribute((const void*)k_z_op<R, MODE, NZ, G, TL>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  attr = true;
-  hipLaunchKernelGGL((k_z_op<R, MODE, NZ, G, TL>), dim3(gridx, njobs), dim3(C::T), lds, st, d, step);
+  if constexpr (D) {
+    hipLaunchKernelGGL((k_z_direct<R, MODE, NZ, G, TL>), dim3(gridx, njobs), dim3(C::T), lds, st, d, step);
+  } else {
+    if (!attr && lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)k_z_op<R, MODE, NZ, G, TL>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+    hipLaunchKernelGGL((k_z_op<R, MODE, NZ, G, TL>), dim3(gridx, njobs), dim3(C::T), lds, st, d, step);
+  }
   return hipGetLastError();
 }
 
-template <typename R, int NZ, int G, int TL>
+template <typename R, int NZ, int G, int TL, int D>
 static hipError_t launch_z_t(int mode, const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
-  if (mode == MODE_CONV) return launch_z_m<R, MODE_CONV, NZ, G, TL>(d, njobs, gridx, step, st);
-  if (mode == MODE_FWD) return launch_z_m<R, MODE_FWD, NZ, G, TL>(d, njobs, gridx, step, st);
-  return launch_z_m<R, MODE_INV, NZ, G, TL>(d, njobs, gridx, step, st);
+  if (mode == MODE_CONV) return launch_z_m<R, MODE_CONV, NZ, G, TL, D>(d, njobs, gridx, step, st);
+  if (mode == MODE_FWD) return launch_z_m<R, MODE_FWD, NZ, G, TL, D>(d, njobs, gridx, step, st);
+  return launch_z_m<R, MODE_INV, NZ, G, TL, D>(d, njobs, gridx, step, st);
 }
 
 int fast_plane_variants(int prec, int nx, int ny, int* ids, int max_ids) {
   int n = 0, id = 0;
+  // lane groups holding more than 256 bytes of a line run out of registers
 #define X(a, b, ga, gb, dd)                                                                         \
-  if (nx == a && ny == b && n < max_ids &&                                                          \
+  if (nx == a && ny == b && n < max_ids && ((a / ga) * (prec == 0 ? 16 : 8)) <= 256 &&            \
       (prec == 0 ? plane_lds_bytes<double, a, b, ga, gb>() : plane_lds_bytes<float, a, b, ga, gb>()) <= \
           160 * 1024)                                                                               \
     ids[n++] = id;                                                                                  \
@@ -316,9 +391,10 @@ int fast_plane_variants(int prec, int nx, int ny, int* ids, int max_ids) {
 
 int fast_z_variants(int prec, int nz, int* ids, int* tiles, int max_ids) {
   int n = 0, id = 0;
-#define X(a, g, tl)                                                                                 \
-  if (nz == a && n < max_ids &&                                                                     \
-      (prec == 0 ? z_lds_bytes<double, a, g, tl>() : z_lds_bytes<float, a, g, tl>()) <= 160 * 1024) { \
+#define X(a, g, tl, dd)                                                                             \
+  if (nz == a && n < max_ids && ((a / g) * (prec == 0 ? 16 : 8)) <= 256 &&                        \
+      (prec == 0 ? z_variant_lds<double, a, g, tl, dd>() : z_variant_lds<float, a, g, tl, dd>()) <=   \
+          160 * 1024) {                                                                             \
     tiles[n] = tl;                                                                                  \
     ids[n++] = id;                                                                                  \
   }                                                                                                 \
@@ -344,10 +420,10 @@ hipError_t launch_fast_plane(int prec, int mode, int variant, const PassDesc* d,
 hipError_t launch_fast_z(int prec, int mode, int variant, const PassDesc* d, int njobs, int gridx, int step,
                          hipStream_t st) {
   int id = 0;
-#define X(a, g, tl)                                                                          \
+#define X(a, g, tl, dd)                                                                      \
   if (variant == id)                                                                         \
-    return prec == 0 ? launch_z_t<double, a, g, tl>(mode, d, njobs, gridx, step, st)         \
-                     : launch_z_t<float, a, g, tl>(mode, d, njobs, gridx, step, st);         \
+    return prec == 0 ? launch_z_t<double, a, g, tl, dd>(mode, d, njobs, gridx, step, st)     \
+                     : launch_z_t<float, a, g, tl, dd>(mode, d, njobs, gridx, step, st);     \
   id++;
   SCFT_Z_VARIANTS(X)
 #undef X

@@ -283,9 +283,12 @@ __device__ __forceinline__ void group_conv(cpx<R>* v, int t, const cpx<R>* __res
 }
 
 // One line transform in the lane-group layout: MODE 0 convolution,
-// 1 forward DFT, 2 backward DFT.  Point i of the line is read with ld(i)
-// and written with st(i, value); forward output and backward input are in
-// natural frequency order.
+// 1 forward DFT, 2 backward DFT.  Point b + o of the line is read with
+// ld(b, o) and written with st(b, o, value), b the lane's first index (t,
+// or E bitrev(t) in the frequency layout) and o an offset known after
+// unrolling, so accessors can put the lane part in one register and fold
+// the rest into immediate or scalar offsets.  Forward output and backward
+// input are in natural frequency order.
 template <int MODE, typename R, int NL, int G, typename Ld, typename St>
 __device__ __forceinline__ void line_op_io(int t, const cpx<R>* __restrict__ tw, const R* __restrict__ g,
                                            Ld ld, St st) {
@@ -294,22 +297,22 @@ __device__ __forceinline__ void line_op_io(int t, const cpx<R>* __restrict__ tw,
   if (MODE == 2) {
     const int kb = E * bitrev_g<G>(t);
 #pragma unroll
-    for (int k = 0; k < E; k++) v[k] = ld(kb + k);
+    for (int k = 0; k < E; k++) v[k] = ld(kb, k);
     group_inv<R, NL, G>(v, t, tw);
 #pragma unroll
-    for (int m = 0; m < E; m++) st(t + G * m, v[m]);
+    for (int m = 0; m < E; m++) st(t, G * m, v[m]);
   } else {
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = ld(t + G * m);
+    for (int m = 0; m < E; m++) v[m] = ld(t, G * m);
     if (MODE == 0) {
       group_conv<R, NL, G>(v, t, tw, g);
 #pragma unroll
-      for (int m = 0; m < E; m++) st(t + G * m, v[m]);
+      for (int m = 0; m < E; m++) st(t, G * m, v[m]);
     } else {
       group_fwd<R, NL, G>(v, t, tw);
       const int kb = E * bitrev_g<G>(t);
 #pragma unroll
-      for (int k = 0; k < E; k++) st(kb + k, v[k]);
+      for (int k = 0; k < E; k++) st(kb, k, v[k]);
     }
   }
 }
@@ -319,7 +322,49 @@ __device__ __forceinline__ void line_op_io(int t, const cpx<R>* __restrict__ tw,
 template <int MODE, typename R, int NL, int G, typename At>
 __device__ __forceinline__ void line_op(int t, const cpx<R>* __restrict__ tw, const R* __restrict__ g, At at) {
   line_op_io<MODE, R, NL, G>(
-      t, tw, g, [&](int i) { return *at(i); }, [&](int i, cpx<R> v) { *at(i) = v; });
+      t, tw, g, [&](int b, int o) { return *at(b + o); }, [&](int b, int o, cpx<R> v) { *at(b + o) = v; });
+}
+
+// ---------------------------------------------------------------------------
+// Raw buffer access (MUBUF) to one array of complex values: the base is a
+// wave-uniform resource, the lane part of the address one 32-bit register
+// (voff) and the rest a scalar offset (soff), all in bytes.  Accesses past
+// `bytes` read zero and drop writes.
+// ---------------------------------------------------------------------------
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, unsigned bytes) {
+  // word 3 = 0x00020000: the raw-buffer format of the gfx9 family
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename R>
+__device__ __forceinline__ cpx<R> buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ cpx<float> buf_ld<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const u32x2_t u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  return mk(__uint_as_float(u.x), __uint_as_float(u.y));
+}
+template <>
+__device__ __forceinline__ cpx<double> buf_ld<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const u32x4_t u = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  return mk(__hiloint2double((int)u.y, (int)u.x), __hiloint2double((int)u.w, (int)u.z));
+}
+
+__device__ __forceinline__ void buf_st(cpx<float> v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  u32x2_t u;
+  u.x = __float_as_uint(v.x);
+  u.y = __float_as_uint(v.y);
+  __builtin_amdgcn_raw_buffer_store_b64(u, r, voff, soff, 0);
+}
+__device__ __forceinline__ void buf_st(cpx<double> v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  u32x4_t u;
+  u.x = (unsigned)__double2loint(v.x);
+  u.y = (unsigned)__double2hiint(v.x);
+  u.z = (unsigned)__double2loint(v.y);
+  u.w = (unsigned)__double2hiint(v.y);
+  __builtin_amdgcn_raw_buffer_store_b128(u, r, voff, soff, 0);
 }
 
 }  // namespace scft
