@@ -144,6 +144,12 @@ struct scft_ctx {
   int prec = 0;
   size_t esz = 16;  // bytes per propagator complex
   hipStream_t st = nullptr;
+  // second stream of the q-dagger chain: the two chains of the propagator
+  // run as parallel branches (fork / join events), so one chain's launches
+  // fill the other's gaps and tails
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool split_chains = false;
   std::vector<SysHost> sys;
   SysDev* d_sys = nullptr;
   int maxM = 0, maxN = 0, nblk = 1;
@@ -152,6 +158,7 @@ struct scft_ctx {
   void* tw32 = nullptr;
   std::vector<void*> allocs;
   Program prog_smear, prog_prop, prog_fwd, prog_inv, prog_noise;
+  Program prog_prop_q, prog_prop_qd;  // the two chains of prog_prop, one each
   std::vector<Program> prog_fft_fwd, prog_fft_inv;  // per system, on its scratch
   std::vector<void*> scratch;                       // per system FP64 scratch
   hipGraphExec_t graph = nullptr;
@@ -355,15 +362,16 @@ int build_program(scft_ctx* c, int mode, const std::vector<JobSpec>& jobs, Progr
   return SCFT_OK;
 }
 
-hipError_t launch_one(scft_ctx* c, const Launch& L, int step) {
+hipError_t launch_one(scft_ctx* c, const Launch& L, int step, hipStream_t st = nullptr) {
+  if (!st) st = c->st;
   if (L.fast == 1)
-    return scft::launch_fast_plane(L.prec, L.mode, L.variants[L.choice], L.d, L.njobs, L.gridx, step, c->st);
+    return scft::launch_fast_plane(L.prec, L.mode, L.variants[L.choice], L.d, L.njobs, L.gridx, step, st);
   if (L.fast == 2) {
     const int tl = L.tiles[L.choice];
     return scft::launch_fast_z(L.prec, L.mode, L.variants[L.choice], L.d, L.njobs, (L.max_lines + tl - 1) / tl,
-                               step, c->st);
+                               step, st);
   }
-  return scft::launch_pass(L.prec, L.mode, L.plane, L.d, L.njobs, L.gridx, L.tb, L.lds, step, c->st);
+  return scft::launch_pass(L.prec, L.mode, L.plane, L.d, L.njobs, L.gridx, L.tb, L.lds, step, st);
 }
 
 // Autotuner: time every variant of each size-specialised launch of a
@@ -400,8 +408,8 @@ int autotune(scft_ctx* c, Program& p, int step) {
   return SCFT_OK;
 }
 
-int run_program(scft_ctx* c, const Program& p, int step) {
-  for (const Launch& L : p.launches) HIPCK(launch_one(c, L, step));
+int run_program(scft_ctx* c, const Program& p, int step, hipStream_t st = nullptr) {
+  for (const Launch& L : p.launches) HIPCK(launch_one(c, L, step, st));
   return SCFT_OK;
 }
 
@@ -822,7 +830,7 @@ JobSpec job(int sys, int prec, void* in, void* out, int gkind, double scale) {
 
 int build_programs(scft_ctx* c) {
   const int ns = (int)c->sys.size();
-  std::vector<JobSpec> smear, prop, fwd, inv, noise;
+  std::vector<JobSpec> smear, prop, prop_q, prop_qd, fwd, inv, noise;
   for (int i = 0; i < ns; i++) {
     SysHost& s = c->sys[i];
     SysDev& d = s.dev;
@@ -838,6 +846,7 @@ int build_programs(scft_ctx* c) {
     q.post_switch = s.Na > 0 ? s.Na : 0;
     q.steps_active = s.N;
     prop.push_back(q);
+    prop_q.push_back(q);
     if (d.qd != d.q) {
       JobSpec qd = q;
       qd.in = qd.out = d.qd;
@@ -845,6 +854,7 @@ int build_programs(scft_ctx* c) {
       qd.post_b = d.expA;
       qd.post_switch = s.N - s.Na;
       prop.push_back(qd);
+      prop_qd.push_back(qd);
     }
     const double inv_m = 1.0 / double(s.M);
     for (void* a : {(void*)d.rhoA, (void*)d.rhoB, (void*)d.wpl, (void*)d.wabm, (void*)d.wabp})
@@ -859,6 +869,11 @@ int build_programs(scft_ctx* c) {
   int rc;
   if ((rc = build_program(c, scft::MODE_CONV, smear, &c->prog_smear))) return rc;
   if ((rc = build_program(c, scft::MODE_CONV, prop, &c->prog_prop))) return rc;
+  c->split_chains = !prop_qd.empty() && !getenv("SCFT_ONE_STREAM");
+  if (c->split_chains) {
+    if ((rc = build_program(c, scft::MODE_CONV, prop_q, &c->prog_prop_q))) return rc;
+    if ((rc = build_program(c, scft::MODE_CONV, prop_qd, &c->prog_prop_qd))) return rc;
+  }
   if ((rc = build_program(c, scft::MODE_FWD, fwd, &c->prog_fwd))) return rc;
   if ((rc = build_program(c, scft::MODE_INV, inv, &c->prog_inv))) return rc;
   if (!noise.empty() && (rc = build_program(c, scft::MODE_FWD, noise, &c->prog_noise))) return rc;
@@ -884,9 +899,20 @@ int stage_poly_density(scft_ctx* c) {
   if (rc) return rc;
   if (c->prec == 1) HIPCK(scft::launch_reduce(scft::RED_SHIFT, 0, c->d_sys, ns, c->nblk, c->st));
   HIPCK(scft::launch_exp_prop(c->prec, c->d_sys, ns, c->maxM, c->st));
-  for (int step = 1; step < c->maxN; step++) {
-    rc = run_program(c, c->prog_prop, step);
-    if (rc) return rc;
+  if (c->split_chains) {
+    HIPCK(hipEventRecord(c->ev_fork, c->st));
+    HIPCK(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+    for (int step = 1; step < c->maxN; step++) {
+      if ((rc = run_program(c, c->prog_prop_q, step, c->st))) return rc;
+      if ((rc = run_program(c, c->prog_prop_qd, step, c->st2))) return rc;
+    }
+    HIPCK(hipEventRecord(c->ev_join, c->st2));
+    HIPCK(hipStreamWaitEvent(c->st, c->ev_join, 0));
+  } else {
+    for (int step = 1; step < c->maxN; step++) {
+      rc = run_program(c, c->prog_prop, step);
+      if (rc) return rc;
+    }
   }
   HIPCK(scft::launch_reduce(scft::RED_Q, c->prec, c->d_sys, ns, c->nblk, c->st));
   HIPCK(scft::launch_density(c->prec, c->d_sys, ns, c->maxM, c->st));
@@ -962,8 +988,11 @@ int scft_create(const scft_params* params, int n_systems, int device, int precis
   c->sys.resize(n_systems);
   c->scratch.resize(n_systems);
   int rc = SCFT_OK;
-  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    scft_destroy(c);
     return fail(SCFT_EDEVICE, "hipStreamCreate failed");
   }
   for (int i = 0; i < n_systems && rc == SCFT_OK; i++) {
@@ -1013,6 +1042,8 @@ int scft_create(const scft_params* params, int n_systems, int device, int precis
         rc = fail(SCFT_EDEVICE, "memset failed");
     }
     if (rc == SCFT_OK) rc = autotune(c, c->prog_prop, 1);
+    if (rc == SCFT_OK && c->split_chains) rc = autotune(c, c->prog_prop_q, 1);
+    if (rc == SCFT_OK && c->split_chains) rc = autotune(c, c->prog_prop_qd, 1);
     if (rc == SCFT_OK) rc = autotune(c, c->prog_smear, 0);
     // the field programs work in place on w+, wAB+-, rhoA, rhoB: keep a copy
     std::vector<void*> saved;
@@ -1056,11 +1087,15 @@ int scft_destroy(scft_ctx* c) {
   if (!c) return SCFT_OK;
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->st2) (void)hipStreamSynchronize(c->st2);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   for (auto& s : c->sys)
     for (void* p : s.allocs) (void)hipFree(p);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->st) (void)hipStreamDestroy(c->st);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   delete c;
   return SCFT_OK;
 }
@@ -1312,7 +1347,8 @@ int scft_time_kernel(scft_ctx* c, int which, int reps, double* avg_ms, double* b
   double b = 0.0;
   auto run_once = [&]() -> int {
     if (which == 0 || which == 1) {
-      const Program& p = c->prog_prop;
+      // the launch the iteration runs: one chain when the chains are split
+      const Program& p = c->split_chains ? c->prog_prop_q : c->prog_prop;
       const size_t k = which == 0 ? 0 : 1;
       if (k >= p.launches.size()) return fail(SCFT_EINVAL, "this grid has no separate z pass");
       HIPCK(launch_one(c, p.launches[k], 1));
@@ -1336,7 +1372,7 @@ int scft_time_kernel(scft_ctx* c, int which, int reps, double* avg_ms, double* b
   for (auto& s : c->sys) {
     if (which == 4) break;
     const double M = s.M, pe = (double)c->esz;
-    const int chains = (s.dev.qd != s.dev.q) ? 2 : 1;
+    const int chains = (s.dev.qd != s.dev.q && !c->split_chains) ? 2 : 1;
     const bool two_pass = s.nz > 1;
     if (which == 0) b += chains * M * pe * (two_pass ? 2.0 : 3.0);
     else if (which == 1) b += chains * M * pe * 3.0;

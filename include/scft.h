@@ -203,7 +203,9 @@ int scft_synchronize(scft_ctx* ctx);
  * times `reps` back-to-back launches of one kernel of the hot path with HIP
  * events on the context's stream and returns the average duration and the
  * algorithmic HBM bytes one launch moves (each array read or written once).
- *   which = 0: propagator xy-plane pass (all chains, one contour step)
+ *   which = 0: propagator xy-plane pass (one contour step of the q chains
+ *              of all systems; of both chains where they share a launch,
+ *              i.e. when the environment variable SCFT_ONE_STREAM is set)
  *   which = 1: propagator z-pencil pass with exp(-w) applied on store
  *   which = 2: q.q-dagger density sum
  *   which = 3: one full 1S iteration (update + densities), in ms, bytes per
