@@ -49,13 +49,13 @@ __global__ __launch_bounds__((PlaneCfg<NX, NY, GX, GY>::T)) void k_plane_op(
   R* gx = reinterpret_cast<R*>(twy + NY);
   R* gy = gx + NX;
   const PassDesc& d = descs[blockIdx.y];
-  if (d.steps_active && step >= d.steps_active) return;
+  if (d.steps_active && step_contour(step) >= d.steps_active) return;
   const int z = blockIdx.x;
   if (z >= d.nz) return;
   const int tid = threadIdx.x;
   const long poff = (long)z * (NX * NY);
-  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step + d.in_soff) * d.in_step + poff;
-  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step + d.out_soff) * d.out_step + poff;
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step_slice(step) + d.in_soff) * d.in_step + poff;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step_slice(step) + d.out_soff) * d.out_step + poff;
   if (tid < NX) {
     twx[tid] = reinterpret_cast<const cpx<R>*>(d.tw0)[tid];
     if (MODE == 0) gx[tid] = reinterpret_cast<const R*>(d.g0)[tid];
@@ -89,7 +89,7 @@ __global__ __launch_bounds__((PlaneCfg<NX, NY, GX, GY>::T)) void k_plane_op(
   const cpx<R>* post = nullptr;
   R scale = R(1);
   if (d.last) {
-    const void* pp = (step < d.post_switch) ? d.post_a : d.post_b;
+    const void* pp = (step_contour(step) < d.post_switch) ? d.post_a : d.post_b;
     if (pp) post = reinterpret_cast<const cpx<R>*>(pp) + poff;
     scale = R(d.scale);
   }
@@ -121,13 +121,13 @@ __global__ __launch_bounds__((PlaneCfg<NX, NY, GX, GY>::T)) void k_plane_direct(
   R* gx = reinterpret_cast<R*>(twy + NY);
   R* gy = gx + NX;
   const PassDesc& d = descs[blockIdx.y];
-  if (d.steps_active && step >= d.steps_active) return;
+  if (d.steps_active && step_contour(step) >= d.steps_active) return;
   const int z = blockIdx.x;
   if (z >= d.nz) return;
   const int tid = threadIdx.x;
   const long poff = (long)z * (NX * NY);
-  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step + d.in_soff) * d.in_step + poff;
-  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step + d.out_soff) * d.out_step + poff;
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step_slice(step) + d.in_soff) * d.in_step + poff;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step_slice(step) + d.out_soff) * d.out_step + poff;
   if (tid < NX) {
     twx[tid] = reinterpret_cast<const cpx<R>*>(d.tw0)[tid];
     if (MODE == 0) gx[tid] = reinterpret_cast<const R*>(d.g0)[tid];
@@ -154,7 +154,7 @@ __global__ __launch_bounds__((PlaneCfg<NX, NY, GX, GY>::T)) void k_plane_direct(
   const cpx<R>* post = nullptr;
   R scale = R(1);
   if (d.last) {
-    const void* pp = (step < d.post_switch) ? d.post_a : d.post_b;
+    const void* pp = (step_contour(step) < d.post_switch) ? d.post_a : d.post_b;
     if (pp) post = reinterpret_cast<const cpx<R>*>(pp) + poff;
     scale = R(d.scale);
   }
@@ -201,14 +201,14 @@ __global__ __launch_bounds__((ZCfg<NZ, G, TL>::T)) void k_z_op(const PassDesc* _
   cpx<R>* tw = lds + NZ * LP;
   R* g = reinterpret_cast<R*>(tw + NZ);
   const PassDesc& d = descs[blockIdx.y];
-  if (d.steps_active && step >= d.steps_active) return;
+  if (d.steps_active && step_contour(step) >= d.steps_active) return;
   const int tid = threadIdx.x;
   const int nlines = d.nx * d.ny;
   const long S = nlines;
   const int l0 = blockIdx.x * TL;
   if (l0 >= nlines) return;
-  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step + d.in_soff) * d.in_step;
-  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step + d.out_soff) * d.out_step;
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step_slice(step) + d.in_soff) * d.in_step;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step_slice(step) + d.out_soff) * d.out_step;
   for (int i = tid; i < NZ; i += T) {
     tw[i] = reinterpret_cast<const cpx<R>*>(d.tw0)[i];
     if (MODE == 0) g[i] = reinterpret_cast<const R*>(d.g0)[i];
@@ -228,7 +228,7 @@ __global__ __launch_bounds__((ZCfg<NZ, G, TL>::T)) void k_z_op(const PassDesc* _
   const cpx<R>* post = nullptr;
   R scale = R(1);
   if (d.last) {
-    const void* pp = (step < d.post_switch) ? d.post_a : d.post_b;
+    const void* pp = (step_contour(step) < d.post_switch) ? d.post_a : d.post_b;
     if (pp) post = reinterpret_cast<const cpx<R>*>(pp);
     scale = R(d.scale);
   }
@@ -258,7 +258,7 @@ __global__ __launch_bounds__((ZCfg<NZ, G, TL>::T)) void k_z_direct(const PassDes
   cpx<R>* tw = reinterpret_cast<cpx<R>*>(smem_raw);
   R* g = reinterpret_cast<R*>(tw + NZ);
   const PassDesc& d = descs[blockIdx.y];
-  if (d.steps_active && step >= d.steps_active) return;
+  if (d.steps_active && step_contour(step) >= d.steps_active) return;
   const int tid = threadIdx.x;
   const int nlines = d.nx * d.ny;
   const long S = nlines;
@@ -274,12 +274,12 @@ __global__ __launch_bounds__((ZCfg<NZ, G, TL>::T)) void k_z_direct(const PassDes
   if (l >= nlines) return;
   constexpr int SZ = (int)sizeof(cpx<R>);
   const unsigned AB = (unsigned)(S * NZ * SZ);
-  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step + d.in_soff) * d.in_step;
-  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step + d.out_soff) * d.out_step;
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step_slice(step) + d.in_soff) * d.in_step;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step_slice(step) + d.out_soff) * d.out_step;
   const cpx<R>* post = nullptr;
   R scale = R(1);
   if (d.last) {
-    const void* pp = (step < d.post_switch) ? d.post_a : d.post_b;
+    const void* pp = (step_contour(step) < d.post_switch) ? d.post_a : d.post_b;
     if (pp) post = reinterpret_cast<const cpx<R>*>(pp);
     scale = R(d.scale);
   }

@@ -31,6 +31,11 @@ hipError_t launch_scale_real(double* a, long n, double f, hipStream_t st);
 hipError_t launch_noise(const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_noise_advance(const SysDev* s, int nsys, hipStream_t st);
 hipError_t launch_cl_accumulate(const SysDev* s, int nsys, int maxM, hipStream_t st);
+// checkpointed propagators: density of one segment, slice copies
+hipError_t launch_density_seg(int prec, const SysDev* s, int nsys, int maxM, int lo, int n, int qd0, int first,
+                              int last, hipStream_t st);
+hipError_t launch_ck_copy(const SysDev* s, int nsys, int maxM, int esz, int src, int src_slot, int dst, int dst_slot,
+                          int dst2, int dst2_slot, hipStream_t st);
 hipError_t launch_update(int scheme, const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_reduce(int what, int prec, const SysDev* s, int nsys, int nblk, hipStream_t st);
 

@@ -159,6 +159,14 @@ struct scft_ctx {
   std::vector<void*> allocs;
   Program prog_smear, prog_prop, prog_fwd, prog_inv, prog_noise;
   Program prog_prop_q, prog_prop_qd;  // the two chains of prog_prop, one each
+  // checkpointed propagators (SCFT_OPT_CHECKPOINT): segment e covers the q
+  // indices ck_lo[e] .. ck_lo[e] + ck_n[e] - 1 counted from the top
+  // (e = 0 ends at N - 1); prog_ck_q / prog_ck_qd step the two windows
+  bool ckpt = false;
+  int ck_K = 0, ck_ne = 0;
+  std::vector<int> ck_lo, ck_n;
+  std::vector<hipEvent_t> ck_fork, ck_join;
+  Program prog_ck_q, prog_ck_qd;
   std::vector<Program> prog_fft_fwd, prog_fft_inv;  // per system, on its scratch
   std::vector<void*> scratch;                       // per system FP64 scratch
   hipGraphExec_t graph = nullptr;
@@ -362,8 +370,11 @@ int build_program(scft_ctx* c, int mode, const std::vector<JobSpec>& jobs, Progr
   return SCFT_OK;
 }
 
-hipError_t launch_one(scft_ctx* c, const Launch& L, int step, hipStream_t st = nullptr) {
+// `step` is the slice index of the arrays; the contour index is
+// step + contour_off (they differ only in the checkpointed propagator)
+hipError_t launch_one(scft_ctx* c, const Launch& L, int step, hipStream_t st = nullptr, int contour_off = 0) {
   if (!st) st = c->st;
+  step = scft::pack_step(step, step + contour_off);
   if (L.fast == 1)
     return scft::launch_fast_plane(L.prec, L.mode, L.variants[L.choice], L.d, L.njobs, L.gridx, step, st);
   if (L.fast == 2) {
@@ -408,8 +419,8 @@ int autotune(scft_ctx* c, Program& p, int step) {
   return SCFT_OK;
 }
 
-int run_program(scft_ctx* c, const Program& p, int step, hipStream_t st = nullptr) {
-  for (const Launch& L : p.launches) HIPCK(launch_one(c, L, step, st));
+int run_program(scft_ctx* c, const Program& p, int step, hipStream_t st = nullptr, int contour_off = 0) {
+  for (const Launch& L : p.launches) HIPCK(launch_one(c, L, step, st, contour_off));
   return SCFT_OK;
 }
 
@@ -676,11 +687,28 @@ int setup_system(scft_ctx* c, int si) {
   ALLOC(hhat_k, m8);
   ALLOC(Xk, m8);
   ALLOC(X2k, m8);
-  ALLOC(q, (size_t)N * M * pe);
-  if (p.Nda != 0 && p.Nda != N) {
-    ALLOC(qd, (size_t)N * M * pe);
+  if (c->ckpt) {
+    // windows of K + 1 slices and the checkpoints; q_0 and q-dagger_0 are
+    // written to slice 1 of their windows
+    if (p.Nda == 0 || p.Nda == N)
+      return fail(SCFT_EUNSUPPORTED, "checkpointed propagators need 0 < Nda < N");
+    if ((M * pe) % 16 != 0) return fail(SCFT_EUNSUPPORTED, "checkpointed propagators need an even grid size");
+    const int K = (int)std::ceil(std::sqrt((double)N));
+    const int ne = (N + K - 1) / K;
+    ALLOC(ck_sq, (size_t)(K + 1) * M * pe);
+    ALLOC(ck_tq, (size_t)(K + 1) * M * pe);
+    ALLOC(ck_cq, (size_t)ne * M * pe);
+    d.q = (char*)d.ck_sq + (size_t)M * pe;
+    d.qd = (char*)d.ck_tq + (size_t)M * pe;
+    d.qlast = (char*)d.ck_sq + (size_t)std::min(K, N) * M * pe;
   } else {
-    d.qd = d.q;
+    ALLOC(q, (size_t)N * M * pe);
+    if (p.Nda != 0 && p.Nda != N) {
+      ALLOC(qd, (size_t)N * M * pe);
+    } else {
+      d.qd = d.q;
+    }
+    d.qlast = (char*)d.q + (size_t)(N - 1) * M * pe;
   }
   ALLOC(expA, (size_t)M * pe);
   ALLOC(expB, (size_t)M * pe);
@@ -868,8 +896,33 @@ int build_programs(scft_ctx* c) {
   }
   int rc;
   if ((rc = build_program(c, scft::MODE_CONV, smear, &c->prog_smear))) return rc;
-  if ((rc = build_program(c, scft::MODE_CONV, prop, &c->prog_prop))) return rc;
-  c->split_chains = !prop_qd.empty() && !getenv("SCFT_ONE_STREAM");
+  if (!c->ckpt && (rc = build_program(c, scft::MODE_CONV, prop, &c->prog_prop))) return rc;
+  if (c->ckpt) {
+    // chain steps inside the windows: slice u from slice u - 1
+    std::vector<JobSpec> ckq, ckqd;
+    for (int i = 0; i < ns; i++) {
+      SysHost& s = c->sys[i];
+      SysDev& d = s.dev;
+      JobSpec q = job(i, c->prec, d.ck_sq, d.ck_sq, 1, 1.0);
+      q.in_step = q.out_step = s.M;
+      q.in_soff = -1;
+      q.out_soff = 0;
+      q.post_a = d.expA;
+      q.post_b = d.expB;
+      q.post_switch = s.Na;
+      q.steps_active = 0;
+      ckq.push_back(q);
+      JobSpec qd = q;
+      qd.in = qd.out = d.ck_tq;
+      qd.post_a = d.expB;
+      qd.post_b = d.expA;
+      qd.post_switch = s.N - s.Na;
+      ckqd.push_back(qd);
+    }
+    if ((rc = build_program(c, scft::MODE_CONV, ckq, &c->prog_ck_q))) return rc;
+    if ((rc = build_program(c, scft::MODE_CONV, ckqd, &c->prog_ck_qd))) return rc;
+  }
+  c->split_chains = !prop_qd.empty() && !getenv("SCFT_ONE_STREAM") && !c->ckpt;
   if (c->split_chains) {
     if ((rc = build_program(c, scft::MODE_CONV, prop_q, &c->prog_prop_q))) return rc;
     if ((rc = build_program(c, scft::MODE_CONV, prop_qd, &c->prog_prop_qd))) return rc;
@@ -892,6 +945,60 @@ int build_programs(scft_ctx* c) {
 // ---------------------------------------------------------------------------
 // stages
 // ---------------------------------------------------------------------------
+// Checkpointed propagators and densities.  Array ids of launch_ck_copy: 0 q
+// window, 1 q-dagger window, 2 checkpoints.  Slice u of a window holds the
+// chain at contour index first + u - 1 of its segment.
+int stage_ck_density(scft_ctx* c) {
+  const int ns = (int)c->sys.size();
+  const int K = c->ck_K, ne = c->ck_ne, esz = (int)c->esz;
+  int rc = SCFT_OK;
+  auto copy = [&](hipStream_t st, int src, int ss, int dst, int ds, int dst2, int ds2) -> int {
+    HIPCK(scft::launch_ck_copy(c->d_sys, ns, c->maxM, esz, src, ss, dst, ds, dst2, ds2, st));
+    return SCFT_OK;
+  };
+  auto steps = [&](const Program& p, hipStream_t st, int u0, int n, int first) -> int {
+    for (int u = u0; u <= n; u++)
+      if ((rc = run_program(c, p, u, st, first - 1))) return rc;
+    return SCFT_OK;
+  };
+  // forward q pass from the bottom segment (q_0 in slice 1, saved as the
+  // last checkpoint); each segment's top value is carried into slice 0 and
+  // kept as the checkpoint of the segment above
+  for (int e = ne - 1; e >= 0; e--) {
+    const int lo = c->ck_lo[e], n = c->ck_n[e];
+    if (e == ne - 1) {
+      if ((rc = copy(c->st, 0, 1, 2, e, -1, 0)) || (rc = steps(c->prog_ck_q, c->st, 2, n, lo))) return rc;
+    } else if ((rc = steps(c->prog_ck_q, c->st, 1, n, lo))) {
+      return rc;
+    }
+    if (e > 0 && (rc = copy(c->st, 0, n, 0, 0, 2, e - 1))) return rc;
+  }
+  HIPCK(scft::launch_reduce(scft::RED_Q, c->prec, c->d_sys, ns, c->nblk, c->st));
+  // density pass from the top segment: the q segment is recomputed from its
+  // checkpoint (stream 1) while q-dagger advances by the same number of
+  // steps (stream 2); the segment's density sum follows both
+  for (int e = 0; e < ne; e++) {
+    const int lo = c->ck_lo[e], n = c->ck_n[e];
+    HIPCK(hipEventRecord(c->ck_fork[e], c->st));
+    HIPCK(hipStreamWaitEvent(c->st2, c->ck_fork[e], 0));
+    if (e == ne - 1 && e > 0) {
+      // the bottom segment restarts from q_0 in slice 1
+      if ((rc = copy(c->st, 2, e, 0, 1, -1, 0)) || (rc = steps(c->prog_ck_q, c->st, 2, n, lo))) return rc;
+    } else if (e > 0) {
+      if ((rc = copy(c->st, 2, e, 0, 0, -1, 0)) || (rc = steps(c->prog_ck_q, c->st, 1, n, lo))) return rc;
+    }
+    if (e == 0) {
+      if ((rc = steps(c->prog_ck_qd, c->st2, 2, n, 0))) return rc;
+    } else if ((rc = copy(c->st2, 1, K, 1, 0, -1, 0)) || (rc = steps(c->prog_ck_qd, c->st2, 1, n, e * K))) {
+      return rc;
+    }
+    HIPCK(hipEventRecord(c->ck_join[e], c->st2));
+    HIPCK(hipStreamWaitEvent(c->st, c->ck_join[e], 0));
+    HIPCK(scft::launch_density_seg(c->prec, c->d_sys, ns, c->maxM, lo, n, e * K, e == 0, e == ne - 1, c->st));
+  }
+  return SCFT_OK;
+}
+
 int stage_poly_density(scft_ctx* c) {
   const int ns = (int)c->sys.size();
   HIPCK(scft::launch_assemble(c->d_sys, ns, c->maxM, c->st));
@@ -899,6 +1006,7 @@ int stage_poly_density(scft_ctx* c) {
   if (rc) return rc;
   if (c->prec == 1) HIPCK(scft::launch_reduce(scft::RED_SHIFT, 0, c->d_sys, ns, c->nblk, c->st));
   HIPCK(scft::launch_exp_prop(c->prec, c->d_sys, ns, c->maxM, c->st));
+  if (c->ckpt) return stage_ck_density(c);
   if (c->split_chains) {
     HIPCK(hipEventRecord(c->ev_fork, c->st));
     HIPCK(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
@@ -974,6 +1082,8 @@ const char* scft_last_error(void) { return g_err.c_str(); }
 
 int scft_create(const scft_params* params, int n_systems, int device, int precision, scft_ctx** out) {
   if (!params || !out || n_systems < 1) return fail(SCFT_EINVAL, "bad arguments to scft_create");
+  const bool ckpt = (precision & SCFT_OPT_CHECKPOINT) != 0;
+  precision &= ~SCFT_OPT_CHECKPOINT;
   if (precision != SCFT_PREC_FP64 && precision != SCFT_PREC_FP32) return fail(SCFT_EINVAL, "bad precision");
   *out = nullptr;
   int ndev = 0;
@@ -987,7 +1097,32 @@ int scft_create(const scft_params* params, int n_systems, int device, int precis
   c->esz = precision == SCFT_PREC_FP64 ? 16 : 8;
   c->sys.resize(n_systems);
   c->scratch.resize(n_systems);
+  c->ckpt = ckpt;
   int rc = SCFT_OK;
+  if (ckpt) {
+    // one segment plan for the context: every system has the same N
+    const int N = params[0].Nda + params[0].Ndb;
+    for (int i = 0; i < n_systems; i++)
+      if (params[i].Nda + params[i].Ndb != N) {
+        delete c;
+        return fail(SCFT_EUNSUPPORTED, "checkpointed propagators need the same N in all systems");
+      }
+    c->ck_K = (int)std::ceil(std::sqrt((double)N));
+    c->ck_ne = (N + c->ck_K - 1) / c->ck_K;
+    for (int e = 0; e < c->ck_ne; e++) {
+      const int hi = N - 1 - e * c->ck_K, lo = std::max(0, N - (e + 1) * c->ck_K);
+      c->ck_lo.push_back(lo);
+      c->ck_n.push_back(hi - lo + 1);
+    }
+    c->ck_fork.assign(c->ck_ne, nullptr);
+    c->ck_join.assign(c->ck_ne, nullptr);
+    for (int e = 0; e < c->ck_ne; e++)
+      if (hipEventCreateWithFlags(&c->ck_fork[e], hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&c->ck_join[e], hipEventDisableTiming) != hipSuccess) {
+        scft_destroy(c);
+        return fail(SCFT_EDEVICE, "hipEventCreate failed");
+      }
+  }
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -1041,7 +1176,9 @@ int scft_create(const scft_params* params, int n_systems, int device, int precis
       if (rc == SCFT_OK && hipMemsetAsync(s.dev.wb, 0, (size_t)s.M * 16, c->st) != hipSuccess)
         rc = fail(SCFT_EDEVICE, "memset failed");
     }
-    if (rc == SCFT_OK) rc = autotune(c, c->prog_prop, 1);
+    if (rc == SCFT_OK && !c->ckpt) rc = autotune(c, c->prog_prop, 1);
+    if (rc == SCFT_OK && c->ckpt) rc = autotune(c, c->prog_ck_q, 2);
+    if (rc == SCFT_OK && c->ckpt) rc = autotune(c, c->prog_ck_qd, 2);
     if (rc == SCFT_OK && c->split_chains) rc = autotune(c, c->prog_prop_q, 1);
     if (rc == SCFT_OK && c->split_chains) rc = autotune(c, c->prog_prop_qd, 1);
     if (rc == SCFT_OK) rc = autotune(c, c->prog_smear, 0);
@@ -1094,6 +1231,10 @@ int scft_destroy(scft_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->st) (void)hipStreamDestroy(c->st);
   if (c->st2) (void)hipStreamDestroy(c->st2);
+  for (hipEvent_t e : c->ck_fork)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->ck_join)
+    if (e) (void)hipEventDestroy(e);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   delete c;
@@ -1345,6 +1486,7 @@ int scft_time_kernel(scft_ctx* c, int which, int reps, double* avg_ms, double* b
   if (reps < 1 || !avg_ms || !bytes) return fail(SCFT_EINVAL, "bad arguments");
   const int ns = (int)c->sys.size();
   double b = 0.0;
+  if (c->ckpt && which != 3) return fail(SCFT_EINVAL, "with checkpointed propagators only which = 3 is timed");
   auto run_once = [&]() -> int {
     if (which == 0 || which == 1) {
       // the launch the iteration runs: one chain when the chains are split
@@ -1431,9 +1573,30 @@ int scft_describe(scft_ctx* c, char* buf, int len) {
     o += "\n";
   };
   prog("smear", c->prog_smear);
-  prog("prop", c->prog_prop);
+  if (c->ckpt) {
+    prog("prop_checkpointed_q", c->prog_ck_q);
+    prog("prop_checkpointed_qdag", c->prog_ck_qd);
+  } else if (c->split_chains) {
+    prog("prop_q", c->prog_prop_q);
+    prog("prop_qdag", c->prog_prop_qd);
+  } else {
+    prog("prop", c->prog_prop);
+  }
   prog("field_fwd", c->prog_fwd);
   prog("field_inv", c->prog_inv);
+  {
+    // propagator storage of system 0
+    const SysHost& s0 = c->sys[0];
+    const double slice = (double)s0.M * c->esz;
+    const int slices = c->ckpt ? 2 * (c->ck_K + 1) + c->ck_ne : (s0.dev.qd != s0.dev.q ? 2 : 1) * s0.N;
+    char tmp[256];
+    if (c->ckpt)
+      snprintf(tmp, sizeof tmp, "propagator_storage: %d slices, %.1f MiB (checkpointed, K=%d, %d segments)\n",
+               slices, slices * slice / 1048576.0, c->ck_K, c->ck_ne);
+    else
+      snprintf(tmp, sizeof tmp, "propagator_storage: %d slices, %.1f MiB\n", slices, slices * slice / 1048576.0);
+    o += tmp;
+  }
   snprintf(buf, len, "%s", o.c_str());
   return SCFT_OK;
 }

@@ -12,6 +12,15 @@ namespace scft {
 
 enum { MODE_CONV = 0, MODE_FWD = 1, MODE_INV = 2 };
 
+// The `step` argument of a pass: the slice index that addresses the arrays
+// (in = in + (slice + in_soff) * in_step) in the low 16 bits, and the
+// contour index s that selects exp(-w(s)) and the jobs still active in the
+// high 16 bits.  They are equal except in the checkpointed propagator, whose
+// windows hold one segment of the chain.
+__host__ __device__ constexpr int pack_step(int slice, int contour) { return slice | (contour << 16); }
+__device__ __forceinline__ int step_slice(int step) { return step & 0xFFFF; }
+__device__ __forceinline__ int step_contour(int step) { return step >> 16; }
+
 // One array of one system for an FFT-style pass.
 struct PassDesc {
   const void* in;
@@ -56,6 +65,11 @@ struct SysDev {
   double *Xk;               // phiD rho0 N h^2 (gaa + 2 gab + gbb) (+ graft term)
   double *X2k;              // phiD rho0 N h^2 (gaa - gbb), negative-chi branch
   void *q, *qd;             // N * M propagators, propagator precision
+                            // (checkpointed: slot 1 of the windows below)
+  void *qlast;              // q_{N-1}
+  // checkpointed propagators: windows of K + 1 slices of the q and q-dagger
+  // chains and the q checkpoints (null otherwise)
+  void *ck_sq, *ck_tq, *ck_cq;
   void *expA, *expB;        // exp(-(smw - shift)), propagator precision
   double2 *etap, *etam;     // noise (k-space): etap[0..M) w+, etap[M..2M) AB+, etam AB-
   unsigned long long *rng_ctr;  // iteration counter of the noise generator

@@ -52,14 +52,14 @@ __global__ __launch_bounds__(TB) void k_plane(const PassDesc* __restrict__ descs
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
   const PassDesc& d = descs[blockIdx.y];
-  if (d.steps_active && step >= d.steps_active) return;
+  if (d.steps_active && step_contour(step) >= d.steps_active) return;
   const int z = blockIdx.x;
   if (z >= d.nz) return;
   const int tid = threadIdx.x;
   const int nx = d.nx, ny = d.ny, nxy = nx * ny;
   const long poff = (long)z * nxy;
-  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step + d.in_soff) * d.in_step + poff;
-  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step + d.out_soff) * d.out_step + poff;
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step_slice(step) + d.in_soff) * d.in_step + poff;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step_slice(step) + d.out_soff) * d.out_step + poff;
   for (int i = tid; i < nxy; i += TB) lds[i] = in[i];
   __syncthreads();
   const cpx<R>* tw0 = reinterpret_cast<const cpx<R>*>(d.tw0);
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(TB) void k_plane(const PassDesc* __restrict__ descs
   const cpx<R>* post = nullptr;
   R scale = R(1);
   if (d.last) {
-    const void* pp = (step < d.post_switch) ? d.post_a : d.post_b;
+    const void* pp = (step_contour(step) < d.post_switch) ? d.post_a : d.post_b;
     if (pp) post = reinterpret_cast<const cpx<R>*>(pp) + poff;
     scale = R(d.scale);
   }
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(TB) void k_line(const PassDesc* __restrict__ descs,
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
   const PassDesc& d = descs[blockIdx.y];
-  if (d.steps_active && step >= d.steps_active) return;
+  if (d.steps_active && step_contour(step) >= d.steps_active) return;
   if ((int)blockIdx.x >= d.ntiles) return;
   const int tid = threadIdx.x;
   const int axis = d.axis, nx = d.nx, ny = d.ny, nz = d.nz;
@@ -124,8 +124,8 @@ __global__ __launch_bounds__(TB) void k_line(const PassDesc* __restrict__ descs,
   const int tl = d.tl;
   const int l0 = blockIdx.x * tl;
   const long S = axis == 0 ? 1 : (axis == 1 ? nx : (long)nx * ny);
-  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step + d.in_soff) * d.in_step;
-  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step + d.out_soff) * d.out_step;
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step_slice(step) + d.in_soff) * d.in_step;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step_slice(step) + d.out_soff) * d.out_step;
   const cpx<R>* tw = reinterpret_cast<const cpx<R>*>(d.tw0);
   const R* g = reinterpret_cast<const R*>(d.g0);
   const int ntot = tl * n;
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(TB) void k_line(const PassDesc* __restrict__ descs,
   const cpx<R>* post = nullptr;
   R scale = R(1);
   if (d.last) {
-    const void* pp = (step < d.post_switch) ? d.post_a : d.post_b;
+    const void* pp = (step_contour(step) < d.post_switch) ? d.post_a : d.post_b;
     if (pp) post = reinterpret_cast<const cpx<R>*>(pp);
     scale = R(d.scale);
   }
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void k_exp_prop(const SysDev* __restrict__ sys
 template <int WHAT, typename R>
 __device__ __forceinline__ double2 red_term(const SysDev& s, long i) {
   if constexpr (WHAT == RED_Q) {
-    const cpx<R> v = ld<R>(s.q, (long)(s.N - 1) * s.M + i);
+    const cpx<R> v = ld<R>(s.qlast, i);
     return make_double2((double)v.x, (double)v.y);
   } else if constexpr (WHAT == RED_H) {
     // wpl_part + wab_part integrands, code/calc_h.cpp:70-73, :97-98
@@ -339,6 +339,64 @@ __global__ __launch_bounds__(256) void k_density(const SysDev* __restrict__ sys)
   const double2 eb = dexp(make_double2(wb.x - c, wb.y));
   s.rhoA[i] = dmul(accA, dmul(ea, factor));
   s.rhoB[i] = dmul(accB, dmul(eb, factor));
+}
+
+// Density of one segment of the checkpointed propagators: q_j for
+// j = lo .. lo + n - 1 in slices 1..n of the q window, q-dagger_{N-1-j} in
+// slice N - 1 - j - qd0 + 1 of the q-dagger window (qd0 = first q-dagger
+// index of the window).  The first segment assigns, later ones add; the last
+// applies nD / (Q V) exp(smw - c) like k_density.
+template <typename R>
+__global__ __launch_bounds__(256) void k_density_seg(const SysDev* __restrict__ sys, int lo, int n, int qd0,
+                                                     int first, int last) {
+  const SysDev& s = sys[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.M) return;
+  const cpx<R>* q = reinterpret_cast<const cpx<R>*>(s.ck_sq);
+  const cpx<R>* qd = reinterpret_cast<const cpx<R>*>(s.ck_tq);
+  const long M = s.M;
+  double2 accA = make_double2(0.0, 0.0), accB = make_double2(0.0, 0.0);
+  for (int u = 1; u <= n; u++) {
+    const int j = lo + u - 1;
+    const cpx<R> a = q[(long)u * M + i];
+    const cpx<R> b = qd[(long)(s.N - 1 - j - qd0 + 1) * M + i];
+    const double2 p = make_double2((double)a.x * b.x - (double)a.y * b.y,
+                                   (double)a.x * b.y + (double)a.y * b.x);
+    if (s.Na > 0 && j < s.Na) accA = dadd(accA, p);
+    else accB = dadd(accB, p);
+  }
+  if (!first) {
+    accA = dadd(accA, s.rhoA[i]);
+    accB = dadd(accB, s.rhoB[i]);
+  }
+  if (last) {
+    const double2 Qs = dscale(s.red[RED_Q], s.dV / s.V);
+    const double2 factor = ddiv(make_double2(s.nD, 0.0), dscale(Qs, s.V));
+    const double c = s.red[RED_SHIFT].y;
+    const double2 wa = s.wa[i], wb = s.wb[i];
+    accA = dmul(accA, dmul(dexp(make_double2(wa.x - c, wa.y)), factor));
+    accB = dmul(accB, dmul(dexp(make_double2(wb.x - c, wb.y)), factor));
+  }
+  s.rhoA[i] = accA;
+  s.rhoB[i] = accB;
+}
+
+// Copy of one slice (M propagator values) between the checkpoint arrays of
+// every system: array 0 = q window, 1 = q-dagger window, 2 = checkpoints;
+// an optional second destination (dst2 < 0: none).  n16 = 16-byte units
+// per slice.
+__device__ __forceinline__ float4* ck_slice(const SysDev& s, int arr, int slot, int esz) {
+  char* base = (char*)(arr == 0 ? s.ck_sq : arr == 1 ? s.ck_tq : s.ck_cq);
+  return reinterpret_cast<float4*>(base + (size_t)slot * s.M * esz);
+}
+__global__ __launch_bounds__(256) void k_ck_copy(const SysDev* __restrict__ sys, int esz, int src, int src_slot,
+                                                 int dst, int dst_slot, int dst2, int dst2_slot) {
+  const SysDev& s = sys[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)s.M * esz / 16) return;
+  const float4 v = ck_slice(s, src, src_slot, esz)[i];
+  ck_slice(s, dst, dst_slot, esz)[i] = v;
+  if (dst2 >= 0) ck_slice(s, dst2, dst2_slot, esz)[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -773,6 +831,21 @@ hipError_t launch_scale_real(double* a, long n, double f, hipStream_t st) {
 
 hipError_t launch_noise(const SysDev* s, int nsys, int maxM, hipStream_t st) {
   hipLaunchKernelGGL(k_noise, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_density_seg(int prec, const SysDev* s, int nsys, int maxM, int lo, int n, int qd0, int first,
+                              int last, hipStream_t st) {
+  if (prec == 0) hipLaunchKernelGGL(k_density_seg<double>, pgrid(maxM, nsys), dim3(256), 0, st, s, lo, n, qd0, first, last);
+  else hipLaunchKernelGGL(k_density_seg<float>, pgrid(maxM, nsys), dim3(256), 0, st, s, lo, n, qd0, first, last);
+  return hipGetLastError();
+}
+
+hipError_t launch_ck_copy(const SysDev* s, int nsys, int maxM, int esz, int src, int src_slot, int dst, int dst_slot,
+                          int dst2, int dst2_slot, hipStream_t st) {
+  const long n16 = (long)maxM * esz / 16;
+  hipLaunchKernelGGL(k_ck_copy, dim3((unsigned)((n16 + 255) / 256), nsys), dim3(256), 0, st, s, esz, src, src_slot,
+                     dst, dst_slot, dst2, dst2_slot);
   return hipGetLastError();
 }
 

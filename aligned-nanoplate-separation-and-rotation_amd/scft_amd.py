@@ -34,6 +34,7 @@ SCFT_EIO = -6
 
 PREC_FP64 = 0
 PREC_FP32 = 1
+OPT_CHECKPOINT = 0x100   # OR into the precision: sqrt(N)-checkpointed propagators
 
 NP_ROD = 2
 NP_SQUARE = 3

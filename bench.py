@@ -145,6 +145,12 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     planb = ctxb.describe()
     ctxb.close()
+    # the same batch with sqrt(N)-checkpointed propagators (memory-bounded
+    # mode: about 3 sqrt(N) instead of 2 N slices per system)
+    ctxc, dtc, _ = run_batch(batch, scft_amd.PREC_FP32 | scft_amd.OPT_CHECKPOINT, args.warmup, args.steps)
+    batched_ckpt = nb * args.steps / dtc
+    ckpt_storage = [l for l in ctxc.describe().splitlines() if l.startswith("propagator_storage")]
+    ctxc.close()
 
     # the single production system, FP32 propagator and plain FP64
     w = scft_amd.config(WORKLOAD)
@@ -186,6 +192,8 @@ def main():
         },
         "value_definition": "system-iterations/s of the FP32 batch (16 systems x 1 iteration per step)",
         "single_system_fp32_iter_s": it32,
+        "fp32_batched_checkpointed_iter_s": batched_ckpt,
+        "checkpointed_storage": ckpt_storage[0] if ckpt_storage else None,
         "fp64_plain_iter_s": it64,
         "fp32_batched_over_fp64_plain": batched / it64,
         "fp32_single_over_fp64_plain": it32 / it64,

@@ -36,6 +36,14 @@ extern "C" {
 #define SCFT_PREC_FP64 0       /* plain complex128 everywhere              */
 #define SCFT_PREC_FP32 1       /* complex64 propagator and FFTs; fields,
                                   Q, log Q, H and residual stay FP64       */
+/* Option OR-ed into `precision` of scft_create: checkpointed propagators.
+ * Instead of q and q-dagger at all N contour steps (2 N arrays per system,
+ * code/array_utils.cpp:509-515) the context keeps q at ceil(N/K) checkpoints
+ * (K = ceil(sqrt N)) and two windows of K + 1 steps, about 3 sqrt(N) arrays:
+ * each density evaluation recomputes the q chain segment by segment while
+ * the q-dagger chain advances (3(N-1) steps instead of 2(N-1)).  Needs the
+ * same N in all systems and 0 < Nda < N. */
+#define SCFT_OPT_CHECKPOINT 0x100
 
 /* ---- plate shapes ------------------------------------------------------ */
 #define SCFT_NP_SPHERE 1       /* reference np_type 1 (not on this path)   */

@@ -1,5 +1,6 @@
 """Plate forces and torques at converged saddle points of the 32x32x64, N = 50
-system: d in {0.20, 0.30, 0.40} x theta in {0, 5, 45} deg.
+system: d in {0.20, 0.30, 0.40} x theta in {0, 5, 45} deg, with the
+propagators checkpointed every ceil(sqrt N) = 8 contour steps.
 
 For each geometry the batch holds the centre system and the systems displaced
 by +-delta and +-2 delta in D (delta = 1e-3 Rg) and in theta (0.1 deg), 81
@@ -50,8 +51,10 @@ def test_saddle_point_forces_and_torques():
             ws.append(base_workload(dd, tt))
     ps = [w.params() for w in ws]
     f = seeded_fields(ps[0], 11, period_rg=4.0, amp=1.0, sigma=0.0)
-    say("adjoint: 9 geometries x 9 systems, FP64")
-    ctx = scft_amd.Context(ps)
+    say("adjoint: 9 geometries x 9 systems, FP64, checkpointed propagators")
+    # q checkpointed every K = ceil(sqrt N) contour steps (SCFT_OPT_CHECKPOINT)
+    ctx = scft_amd.Context(ps, precision=scft_amd.PREC_FP64 | scft_amd.OPT_CHECKPOINT)
+    say(" " + [l for l in ctx.describe().splitlines() if l.startswith("propagator_storage")][0])
     for i in range(len(ps)):
         ctx.set_fields(*f, sys=i)
     ctx.calc_poly_density()
