@@ -231,7 +231,11 @@ struct Oracle {
   std::vector<cd> hhat, bond, gaa, gab, gbb, gc;
   std::vector<cd> etap, etam, tmp;
   std::vector<std::vector<cd>> q, qdag;
-  cd Qd = 1.0, Hcur = 0.0;
+  // grafted chains on the explicit plates: graft-site density, chain
+  // propagators, graft density and the log-average Qga_exp
+  std::vector<cd> expl_grafts, rhog_exp;
+  std::vector<std::vector<cd>> qg, qgdag;
+  cd Qd = 1.0, Hcur = 0.0, Qga = 0.0;
   Ran2 rng;
   int iter = 0;
 
@@ -393,14 +397,66 @@ struct Oracle {
       z(*a);
     q.assign(N + 1, std::vector<cd>(M));
     qdag.assign(N + 1, std::vector<cd>(M));
+    z(expl_grafts);
+    z(rhog_exp);
+    if (grafted()) {
+      qg.assign(p.Ng, std::vector<cd>(M));
+      qgdag.assign(p.Ng, std::vector<cd>(M));
+    }
     fft = new Fft3(Nx[0], Nx[1], Nx[2]);
   }
 
-  /* initialize_2, code/initialize.cpp:86-412 (film, channel, field NPs and the
-   * graft-site shape of :160-206 are not on this path). */
+  /* grafted chains on explicit plates (sigma > 0, Ng > 0, rods) */
+  bool grafted() const { return p.n_exp_nr > 0 && p.sigma > 0.0 && p.Ng > 0; }
+
+  /* get_k_alias, code/array_utils.cpp:181-230: the Nyquist mode of an even
+   * axis gets k = 0 */
+  void get_k_alias(long id, double k[3]) const {
+    int n[3];
+    unstack(id, n);
+    for (int d = 0; d < 3; d++) {
+      if (d >= Dim) { k[d] = 0.0; continue; }
+      if (Nx[d] % 2 == 0 && n[d] == Nx[d] / 2) k[d] = 0.0;
+      else if (double(n[d]) < double(Nx[d]) / 2.0) k[d] = 2 * kPi * double(n[d]) / L[d];
+      else k[d] = 2 * kPi * double(n[d] - Nx[d]) / L[d];
+    }
+  }
+  /* field_gradient, code/array_utils.cpp:565-579: spectral derivative */
+  void field_gradient(const std::vector<cd>& in, std::vector<cd>& out, int dir) {
+    const cd I(0.0, 1.0);
+    fft_fwd(in.data(), out.data());
+    for (long i = 0; i < M; i++) {
+      double kv[3];
+      get_k_alias(i, kv);
+      out[i] *= I * kv[dir];
+    }
+    fft_bck(out);
+  }
+
+  /* graft sites, code/initialize.cpp:160-196: |grad| of each plate's shape
+   * inflated by 1.05 in length and radius, summed, normalised to unit
+   * integral */
+  void graft_sites() {
+    std::vector<cd> shape(M), g(M), g2(M);
+    for (int np = 0; np < p.n_exp_nr; np++) {
+      plate_density(p.L_nr * 1.05, p.R_nr * 1.05, p.xi_nr, p.exp_nr_c[np], p.exp_nr_u[np], shape);
+      std::fill(g2.begin(), g2.end(), cd(0.0, 0.0));
+      for (int d = 0; d < Dim; d++) {
+        field_gradient(shape, g, d);
+        for (long i = 0; i < M; i++) g2[i] += g[i] * g[i];
+      }
+      for (long i = 0; i < M; i++) expl_grafts[i] += std::sqrt(g2[i]);
+    }
+    const cd total = integ(expl_grafts);
+    for (long i = 0; i < M; i++) expl_grafts[i] /= total;
+  }
+
+  /* initialize_2, code/initialize.cpp:86-412 (film, channel and field NPs are
+   * not on this path). */
   void initialize_2() {
     double k[3];
     for (long i = 0; i < M; i++) hhat[i] = std::exp(-get_k(i, k) * a_sq / 2.0);
+    if (grafted()) graft_sites();
     // plates, :210-239
     if (p.n_exp_nr > 0) {
       std::vector<cd> one(M);
@@ -575,6 +631,40 @@ struct Oracle {
     } else {
       Qd = 1.0;
     }
+    if (grafted()) {
+      graft_free_ends();
+      Qga = grafted_exp_nps();
+    }
+  }
+
+  /* one step of a chain: out = exp(-w) IFFT[bond FFT[in]],
+   * code/grafted_nanoparticle.cpp:29-38 */
+  void chain_step(const std::vector<cd>& in, std::vector<cd>& out, const std::vector<cd>& w) {
+    fft_fwd(in.data(), out.data());
+    for (long i = 0; i < M; i++) out[i] *= bond[i];
+    fft_bck(out);
+    for (long i = 0; i < M; i++) out[i] *= std::exp(-w[i]);
+  }
+  /* graft_homopoly_free_ends, code/grafted_nanoparticle.cpp:15-41 */
+  void graft_free_ends() {
+    for (long i = 0; i < M; i++) qg[0][i] = std::exp(-smwg[i]);
+    for (int j = 1; j < p.Ng; j++) chain_step(qg[j - 1], qg[j], smwg);
+  }
+  /* grafted_exp_nps, code/grafted_nanoparticle.cpp:43-117: the complementary
+   * propagator starts at the graft sites, divided by the free-end
+   * propagator; returns Qga_exp = integ(expl_grafts log q_g[Ng-1]) */
+  cd grafted_exp_nps() {
+    const int Ng = p.Ng;
+    for (long i = 0; i < M; i++) qgdag[0][i] = expl_grafts[i] * std::exp(-smwg[i]) / qg[Ng - 1][i];
+    for (int j = 1; j < Ng; j++) chain_step(qgdag[j - 1], qgdag[j], smwg);
+    for (long i = 0; i < M; i++) {
+      cd r = 0.0;
+      for (int j = 0; j < Ng; j++) r += qg[j][i] * qgdag[Ng - j - 1][i];
+      rhog_exp[i] = r * (ng_per_np * double(p.n_exp_nr) * std::exp(smwg[i]));
+    }
+    for (long i = 0; i < M; i++)
+      tmp[i] = (expl_grafts[i].real() <= 0.0) ? cd(0.0) : expl_grafts[i] * std::log(qg[Ng - 1][i]);
+    return integ(tmp);
   }
 
   /* generate_1s_noise, code/1s_update.cpp:356-368 */
@@ -585,8 +675,48 @@ struct Oracle {
     fft_fwd(et);
   }
 
-  /* update_1s, code/1s_update.cpp:7-352 (homopolymer, graft, field-NP and
-   * channel densities are zero on this path). */
+  /* One graft pair of update_1s: BC (rho = rhodb, g = gbb,
+   * code/1s_update.cpp:162-247) or AC (rho = rhoda, g = gaa, :249-329), for
+   * both signs of chi; the densities are in k-space. */
+  void pair_update(std::vector<cd>& wp, std::vector<cd>& wm, const std::vector<cd>& rho,
+                   const std::vector<cd>& g, double chi, bool neg) {
+    const cd I(0.0, 1.0);
+    const double rho0 = p.rho0, li = p.lam_imag, lr = p.lam_real;
+    const int Ng = p.Ng;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < M; i++) {
+      if (!neg) {
+        cd F = I * hhat[i] * (rhog_exp[i] + rho[i]);
+        cd A = 2.0 * rho0 / chi + phiD * rho0 * N * hhat[i] * hhat[i] * (g[i]) +
+               phiG * rho0 * Ng * hhat[i] * hhat[i] * gc[i];
+        cd B = A - 2.0 * rho0 / chi;
+        cd numer = wp[i] - li * (F - B * wp[i]);
+        if (p.do_CL) numer += etap[i];
+        wp[i] = numer / (1.0 + li * A);
+        F = hhat[i] * (rhog_exp[i] - rho[i]);
+        if (p.np_chem == 1) F += I * rho0 * exp_nrH[i];
+        numer = wm[i] - lr * F;
+        if (p.do_CL) numer += etam[i];
+        wm[i] = numer / (1.0 + lr * 2.0 * rho0 / chi);
+      } else {
+        cd F = -hhat[i] * (rho[i] + rhog_exp[i]);
+        cd numer = wp[i] - lr * F;
+        if (p.do_CL) numer += etap[i];
+        wp[i] = numer / (1.0 + lr * 2.0 * rho0 / chi);
+        F = -I * hhat[i] * (rhog_exp[i] - rho[i]);
+        if (p.np_chem == 1) F += -rho0 * exp_nrH[i];
+        cd A = 2.0 * rho0 / chi + phiD * rho0 * N * hhat[i] * hhat[i] * (g[i]) -
+               phiG * Ng * hhat[i] * hhat[i] * rho0 * gc[i];
+        cd B = A - 2.0 * rho0 / chi;
+        numer = wm[i] - li * (F - B * wm[i]);
+        if (p.do_CL) numer += etap[i];
+        wm[i] = numer / (1.0 + li * A);
+      }
+    }
+  }
+
+  /* update_1s, code/1s_update.cpp:7-352 (homopolymer, field-NP and channel
+   * densities are zero on this path). */
   void update_1s() {
     const cd I(0.0, 1.0);
     const double rho0 = p.rho0, kappa = p.kappa, li = p.lam_imag, lr = p.lam_real;
@@ -594,6 +724,7 @@ struct Oracle {
       fft_fwd(rhoda);
       fft_fwd(rhodb);
     }
+    if (p.sigma > 0.0 && p.n_exp_nr > 0) fft_fwd(rhog_exp);
     fft_fwd(wpl);
     if (chi_ab != 0.0) { fft_fwd(wabm); fft_fwd(wabp); }
     if (chi_bc != 0.0) { fft_fwd(wbcm); fft_fwd(wbcp); }
@@ -603,7 +734,7 @@ struct Oracle {
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < M; i++) {
       cd evar = (i == 0) ? 1.0 : 0.0;
-      cd F = I * rho0 * (surfH[i] + exp_nrH[i] - evar) + I * hhat[i] * (rhoda[i] + rhodb[i]);
+      cd F = I * rho0 * (surfH[i] + exp_nrH[i] - evar) + I * hhat[i] * (rhoda[i] + rhog_exp[i] + rhodb[i]);
       cd A = (kappa <= 0.0 ? 0.0 : rho0 / kappa) +
              phiD * rho0 * N * hhat[i] * hhat[i] * (gaa[i] + 2.0 * gab[i] + gbb[i]) +
              (phiG * p.Ng * gc[i]) * rho0 * hhat[i] * hhat[i];
@@ -644,10 +775,19 @@ struct Oracle {
     } else {
       for (long i = 0; i < M; i++) wabp[i] = wabm[i] = 0.0;
     }
-    // BC and AC pairs only carry graft densities (zero here); with chi = 0
-    // the reference zeroes them (:243-247, :325-329).
-    if (chi_bc == 0.0) for (long i = 0; i < M; i++) wbcp[i] = wbcm[i] = 0.0;
-    if (chi_ac == 0.0) for (long i = 0; i < M; i++) wacp[i] = wacm[i] = 0.0;
+    // BC and AC pairs (:162-329): the graft chains against the B and A blocks
+    if (chi_bc != 0.0) {
+      if (p.do_CL) { noise(etap, lr); noise(etam, li); }
+      pair_update(wbcp, wbcm, rhodb, gbb, chi_bc, neg_bc);
+    } else {
+      for (long i = 0; i < M; i++) wbcp[i] = wbcm[i] = 0.0;
+    }
+    if (chi_ac != 0.0) {
+      if (p.do_CL) { noise(etap, lr); noise(etam, li); }
+      pair_update(wacp, wacm, rhoda, gaa, chi_ac, neg_ac);
+    } else {
+      for (long i = 0; i < M; i++) wacp[i] = wacm[i] = 0.0;
+    }
     fft_bck(wpl);
     if (chi_ab != 0.0) { fft_bck(wabm); fft_bck(wabp); }
     if (chi_bc != 0.0) { fft_bck(wbcm); fft_bck(wbcp); }
@@ -706,14 +846,20 @@ struct Oracle {
       tmp[i] = (kappa <= 0.0 ? cd(0.0) : wpl[i] * wpl[i] * rho0 / kappa / 2.0) -
                I * rho0 * (1.0 - rho_surf[i] - rho_exp_nr[i]) * wpl[i];
     cd h = integ(tmp);
-    if (chi_ab != 0.0) {
-      const double sgn = neg_ab ? -1.0 : 1.0;
+    // Hcur = wpl_part + wbc_part + wac_part + wab_part (:13), pair parts
+    // (:89-132) in that order
+    auto pair_part = [&](const std::vector<cd>& a, const std::vector<cd>& b, double chi, bool neg) {
+      const double sgn = neg ? -1.0 : 1.0;
 #pragma omp parallel for schedule(static)
-      for (long i = 0; i < M; i++)
-        tmp[i] = (wabp[i] * wabp[i] + wabm[i] * wabm[i]) * rho0 / chi_ab * sgn;
-      h += integ(tmp);
-    }
+      for (long i = 0; i < M; i++) tmp[i] = (a[i] * a[i] + b[i] * b[i]) * rho0 / chi * sgn;
+      return integ(tmp);
+    };
+    if (chi_bc != 0.0) h += pair_part(wbcp, wbcm, chi_bc, neg_bc);
+    if (chi_ac != 0.0) h += pair_part(wacp, wacm, chi_ac, neg_ac);
+    if (chi_ab != 0.0) h += pair_part(wabp, wabm, chi_ab, neg_ab);
     if (nD > 0.0) h += -nD * std::log(Qd);
+    // chains grafted to the plates (:31-35)
+    if (p.n_exp_nr > 0 && p.sigma > 0.0) h += -double(p.n_exp_nr) * ng_per_np * Qga.real();
     Hcur = h;
     return h;
   }
@@ -849,6 +995,14 @@ static std::vector<cd>* oracle_array(Oracle* o, const char* name) {
   if (s == "gab") return &o->gab;
   if (s == "gbb") return &o->gbb;
   if (s == "gc") return &o->gc;
+  if (s == "wbcp") return &o->wbcp;
+  if (s == "wbcm") return &o->wbcm;
+  if (s == "wacp") return &o->wacp;
+  if (s == "wacm") return &o->wacm;
+  if (s == "smwg") return &o->smwg;
+  if (s == "expl_grafts") return &o->expl_grafts;
+  if (s == "rhog_exp") return &o->rhog_exp;
+  if (s.rfind("qg:", 0) == 0) { int j = atoi(name + 3); if (j >= 0 && j < (int)o->qg.size()) return &o->qg[j]; }
   if (s.rfind("q:", 0) == 0) { int j = atoi(name + 2); if (j >= 0 && j <= o->N) return &o->q[j]; }
   if (s.rfind("qdag:", 0) == 0) { int j = atoi(name + 5); if (j >= 0 && j <= o->N) return &o->qdag[j]; }
   return nullptr;
@@ -895,6 +1049,8 @@ double oracle_get_scalar(void* h, const char* name) {
   if (s == "Rg2") return o->Rg2;
   if (s == "Qd_re") return o->Qd.real();
   if (s == "Qd_im") return o->Qd.imag();
+  if (s == "Qga_re") return o->Qga.real();
+  if (s == "Qga_im") return o->Qga.imag();
   return NAN;
 }
 
