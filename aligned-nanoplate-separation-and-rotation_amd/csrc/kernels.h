@@ -24,6 +24,11 @@ hipError_t launch_copy16(const void* src, void* dst, long n16, hipStream_t st);
 hipError_t launch_assemble(const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_exp_prop(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_density(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);
+// chains grafted to the plates: complementary start, density; graft sites
+hipError_t launch_graft_init(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);
+hipError_t launch_graft_density(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);
+hipError_t launch_ik_mult(const SysDev* one, long M, const double2* in, double2* out, int d, hipStream_t st);
+hipError_t launch_sites_accum(long M, const double2* a, double2* acc, int mode, hipStream_t st);
 hipError_t launch_cavity(const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_plate_grad(const SysDev* one, int nblk, const double* axis, double lq_re, double lq_im,
                              double2* out, hipStream_t st);

@@ -136,6 +136,11 @@ struct SysHost {
   void* bond_g[3];
   void* smear_g[3];
   double2* grad_out = nullptr;  // device, 8 complex
+  // BC / AC pairs and grafts
+  bool pairs = false, grafted = false, has_sites = false;
+  int neg_bc = 0, neg_ac = 0;
+  double chi_bc = 0.0, chi_ac = 0.0;
+  cd expl_total = 0.0;  // integ(expl_grafts) after normalisation (unit)
   std::vector<void*> allocs;
 };
 
@@ -159,6 +164,10 @@ struct scft_ctx {
   std::vector<void*> allocs;
   Program prog_smear, prog_prop, prog_fwd, prog_inv, prog_noise;
   Program prog_prop_q, prog_prop_qd;  // the two chains of prog_prop, one each
+  Program prog_graft_q, prog_graft_qd;  // graft chains (free end, grafted end)
+  bool any_graft = false, any_pairs = false;
+  size_t dev_bytes = 0;  // device memory held by the context
+  int maxNg = 0;
   // checkpointed propagators (SCFT_OPT_CHECKPOINT): segment e covers the q
   // indices ck_lo[e] .. ck_lo[e] + ck_n[e] - 1 counted from the top
   // (e = 0 ends at N - 1); prog_ck_q / prog_ck_qd step the two windows
@@ -180,7 +189,7 @@ int dalloc(scft_ctx* c, std::vector<void*>& list, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return fail(SCFT_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
   list.push_back(*p);
-  (void)c;
+  c->dev_bytes += bytes;
   return SCFT_OK;
 }
 
@@ -444,8 +453,9 @@ void plate_axes(const double* u, double* e1, double* e2) {
 
 // Plate density at grid point (n0, n1, n2): explicit_nanorod
 // (code/initialize.cpp:578-612) for np_type 2, the square plate otherwise.
-double plate_rho(const SysHost& s, int np, const int* nn) {
+double plate_rho(const SysHost& s, int np, const int* nn, double scale = 1.0) {
   const scft_params& p = s.p;
+  const double len = scale * p.L_nr, rad = scale * p.R_nr;
   const int dim = p.dim;
   double x[3], c[3], dr[3] = {0, 0, 0};
   for (int d = 0; d < dim; d++) {
@@ -458,14 +468,14 @@ double plate_rho(const SysHost& s, int np, const int* nn) {
   const double* u = p.exp_nr_u[np];
   double ud = 0;
   for (int d = 0; d < dim; d++) ud += u[d] * dr[d];
-  const double a = std::erfc((std::fabs(ud) - 0.5 * p.L_nr) / p.xi_nr);
+  const double a = std::erfc((std::fabs(ud) - 0.5 * len) / p.xi_nr);
   if (p.np_type == SCFT_NP_SQUARE && dim == 3) {
     double e1[3], e2[3];
     plate_axes(u, e1, e2);
     const double a1 = e1[0] * dr[0] + e1[1] * dr[1] + e1[2] * dr[2];
     const double a2 = e2[0] * dr[0] + e2[1] * dr[1] + e2[2] * dr[2];
-    return 0.125 * a * std::erfc((std::fabs(a1) - p.R_nr) / p.xi_nr) *
-           std::erfc((std::fabs(a2) - p.R_nr) / p.xi_nr);
+    return 0.125 * a * std::erfc((std::fabs(a1) - rad) / p.xi_nr) *
+           std::erfc((std::fabs(a2) - rad) / p.xi_nr);
   }
   double uc;
   if (dim == 2) uc = std::fabs(u[0] * dr[1] - u[1] * dr[0]);
@@ -474,7 +484,7 @@ double plate_rho(const SysHost& s, int np, const int* nn) {
                  c2 = u[0] * dr[1] - u[1] * dr[0];
     uc = std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
   }
-  return 0.25 * a * std::erfc((std::fabs(uc) - p.R_nr) / p.xi_nr);
+  return 0.25 * a * std::erfc((std::fabs(uc) - rad) / p.xi_nr);
 }
 
 // Numerical Recipes ran2 + polar Box-Muller, for ic_flag == -1 starts
@@ -529,10 +539,17 @@ int setup_system(scft_ctx* c, int si) {
   if (p.Nda < 0 || p.Ndb < 0 || p.Nda + p.Ndb < 2) return fail(SCFT_EINVAL, "Nda + Ndb must be >= 2");
   if (p.do_film || p.do_channel || p.do_fld_np || p.phiH > 0 || p.np_type == SCFT_NP_SPHERE)
     return fail(SCFT_EUNSUPPORTED, "films, channels, field nanoparticles, homopolymer and spheres are not on this path");
-  if (p.sigma > 0.0 && p.Ng > 0)
-    return fail(SCFT_EUNSUPPORTED, "grafted chains are not on this path yet");
-  if (p.chi_bc != 0.0 || p.chi_ac != 0.0)
-    return fail(SCFT_EUNSUPPORTED, "BC/AC field pairs carry graft densities only and are not on this path yet");
+  s.grafted = p.sigma > 0.0 && p.Ng > 0 && p.n_exp_nr > 0;
+  s.pairs = s.grafted || p.chi_bc != 0.0 || p.chi_ac != 0.0;
+  s.has_sites = p.np_type == SCFT_NP_ROD && p.n_exp_nr > 0;
+  if (s.grafted && p.np_type != SCFT_NP_ROD)
+    return fail(SCFT_EUNSUPPORTED, "grafted chains are built for rod plates (np_type 2) only");
+  if (s.pairs && (p.update_scheme != 1 || p.do_CL))
+    return fail(SCFT_EUNSUPPORTED, "grafts and BC/AC pairs are built for the 1S update without noise");
+  s.neg_bc = p.chi_bc < 0;
+  s.chi_bc = std::fabs(p.chi_bc);
+  s.neg_ac = p.chi_ac < 0;
+  s.chi_ac = std::fabs(p.chi_ac);
   if (p.do_CL && p.sample_freq < 1) return fail(SCFT_EINVAL, "do_CL needs sample_freq >= 1");
   s.nx = p.Nx[0];
   s.ny = p.Nx[1];
@@ -579,7 +596,13 @@ int setup_system(scft_ctx* c, int si) {
   const double np_frac = V_exp / Vf;
   const double V1 = p.n_exp_nr > 0 ? V_exp / double(p.n_exp_nr) : 0.0;
   const double nP = p.n_exp_nr > 0 ? V_exp / V1 : 0.0;
-  const double phiD = 1.0 - p.phiH - np_frac;
+  // grafts, code/initialize.cpp:331-345
+  double ng_per_np = 0.0, phiG = 0.0;
+  if (p.sigma > 0.0) {
+    ng_per_np = p.dim == 2 ? p.sigma * 2.0 * kPi * p.R_nr : p.sigma * 4.0 * kPi * p.R_nr * p.R_nr;
+    phiG = ng_per_np * nP * p.Ng / p.rho0 / Vf;
+  }
+  const double phiD = 1.0 - p.phiH - np_frac - phiG;
   const double nD = phiD * p.rho0 * Vf / N;
   scft_sysinfo& inf = s.info;
   memset(&inf, 0, sizeof inf);
@@ -591,16 +614,17 @@ int setup_system(scft_ctx* c, int si) {
   inf.V_1_exp_np = V1;
   inf.np_frac = np_frac;
   inf.phiD = phiD;
-  inf.phiG = 0.0;
+  inf.phiG = phiG;
   inf.nP = nP;
-  inf.ng_per_np = 0.0;
+  inf.ng_per_np = ng_per_np;
   inf.nD = nD;
   inf.Rg2 = Rg2;
   inf.fD = fD;
 
   // k-space coefficients: h(k) (code/initialize.cpp:97-102), Debye sums
   // (code/calc_debye.cpp:5-51) folded into X(k) = phiD rho0 N h^2 (gaa + 2 gab + gbb)
-  std::vector<double> hk(M), X(M), X2(M);
+  std::vector<double> hk(M), X(M), X2(M), XA(M), XB(M), XG(M);
+  const double galpha = p.Ng > 0 ? double(p.Ng - 1) / double(N - 1) : 0.0;
   for (int i = 0; i < M; i++) {
     const int n0 = i % s.nx, n1 = (i / s.nx) % s.ny, n2 = i / (s.nx * s.ny);
     double k2 = 0.0;
@@ -625,10 +649,24 @@ int setup_system(scft_ctx* c, int si) {
     hk[i] = h;
     X[i] = phiD * p.rho0 * N * h * h * (gaa + 2.0 * gab + gbb);
     X2[i] = phiD * p.rho0 * N * h * h * (gaa - gbb);
+    XA[i] = phiD * p.rho0 * N * h * h * gaa;
+    XB[i] = phiD * p.rho0 * N * h * h * gbb;
+    // graft Debye function, calc_gd (code/calc_debye.cpp:53-68) with
+    // alpha = (Ng - 1) / (N - 1), code/initialize.cpp:394
+    const double dg = x * galpha;
+    const double gc = dg == 0.0 ? 1.0 : 2.0 * (std::exp(-dg) + dg - 1.0) / dg / dg;
+    XG[i] = phiG * p.rho0 * p.Ng * h * h * gc;
   }
 
   // initial fields, init_fields (code/initialize.cpp:419-565)
   std::vector<cd> wpl(M, 0.0), wabp(M, 0.0), wabm(M, 0.0);
+  std::vector<cd> wbcp, wbcm, wacp, wacm;
+  if (s.pairs) {
+    wbcp.assign(M, 0.0);
+    wbcm.assign(M, 0.0);
+    wacp.assign(M, 0.0);
+    wacm.assign(M, 0.0);
+  }
   {
     const cd I(0.0, 1.0);
     Ran2 rng(-(long)((p.seed % 2147483000ULL) + 1));
@@ -659,6 +697,14 @@ int setup_system(scft_ctx* c, int si) {
           wabp[i] = p.ic_pre[0] * (rng.next() - 0.5) * (s.neg_ab ? cd(1.0) : I);
           wabm[i] = p.ic_pre[0] * (rng.next() - 0.5) * (s.neg_ab ? I : cd(1.0));
         }
+        if (s.chi_bc != 0) {
+          wbcp[i] = p.ic_pre[0] * (rng.next() - 0.5) * (s.neg_bc ? cd(1.0) : I);
+          wbcm[i] = p.ic_pre[0] * (rng.next() - 0.5) * (s.neg_bc ? I : cd(1.0));
+        }
+        if (s.chi_ac != 0) {
+          wacp[i] = p.ic_pre[0] * (rng.next() - 0.5) * (s.neg_ac ? cd(1.0) : I);
+          wacm[i] = p.ic_pre[0] * (rng.next() - 0.5) * (s.neg_ac ? I : cd(1.0));
+        }
         wpl[i] = p.ic_pre[0] * (rng.next() - 0.5) * I;
       }
     }
@@ -672,9 +718,11 @@ int setup_system(scft_ctx* c, int si) {
   void* p_;
   int rc;
 #define ALLOC(field, bytes)                                 \
-  rc = dalloc(c, s.allocs, &p_, (bytes));                   \
-  if (rc) return rc;                                        \
-  d.field = (decltype(d.field))p_;
+  do {                                                      \
+    rc = dalloc(c, s.allocs, &p_, (bytes));                 \
+    if (rc) return rc;                                      \
+    d.field = (decltype(d.field))p_;                        \
+  } while (0)
   ALLOC(wpl, m16);
   ALLOC(wabp, m16);
   ALLOC(wabm, m16);
@@ -687,6 +735,25 @@ int setup_system(scft_ctx* c, int si) {
   ALLOC(hhat_k, m8);
   ALLOC(Xk, m8);
   ALLOC(X2k, m8);
+  if (s.pairs) {
+    ALLOC(wbcp, m16);
+    ALLOC(wbcm, m16);
+    ALLOC(wacp, m16);
+    ALLOC(wacm, m16);
+    ALLOC(XAk, m8);
+    ALLOC(XBk, m8);
+    ALLOC(XGk, m8);
+  }
+  // graft sites exist for every system with rod plates (code/initialize.cpp:
+  // 160-205 computes them whether or not chains are grafted)
+  if (s.has_sites) ALLOC(expl, m16);
+  if (s.grafted) {
+    ALLOC(wg, m16);
+    ALLOC(rhog, m16);
+    ALLOC(qg, (size_t)p.Ng * M * pe);
+    ALLOC(qgd, (size_t)p.Ng * M * pe);
+    ALLOC(expG, (size_t)M * pe);
+  }
   if (c->ckpt) {
     // windows of K + 1 slices and the checkpoints; q_0 and q-dagger_0 are
     // written to slice 1 of their windows
@@ -790,6 +857,25 @@ int setup_system(scft_ctx* c, int si) {
   HIPCK(hipMemcpy(d.wabm, wabm.data(), m16, hipMemcpyHostToDevice));
   HIPCK(hipMemset(d.rhoA, 0, m16));
   HIPCK(hipMemset(d.rhoB, 0, m16));
+  d.neg_bc = s.neg_bc;
+  d.neg_ac = s.neg_ac;
+  d.chi_bc = s.chi_bc;
+  d.chi_ac = s.chi_ac;
+  d.grafted = s.grafted;
+  d.Ng = s.grafted ? p.Ng : 0;
+  d.n_exp = p.n_exp_nr;
+  d.ng_per_np = ng_per_np;
+  if (s.pairs) {
+    HIPCK(hipMemcpy(d.wbcp, wbcp.data(), m16, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(d.wbcm, wbcm.data(), m16, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(d.wacp, wacp.data(), m16, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(d.wacm, wacm.data(), m16, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(d.XAk, XA.data(), m8, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(d.XBk, XB.data(), m8, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(d.XGk, XG.data(), m8, hipMemcpyHostToDevice));
+  }
+  if (s.has_sites) HIPCK(hipMemset(d.expl, 0, m16));
+  if (s.grafted) HIPCK(hipMemset(d.rhog, 0, m16));
 
   // 1D tables along each axis: bond exp(-k^2/6)/n, smear exp(-k^2 a^2/2)/n
   for (int ax = 0; ax < 3; ax++) {
@@ -858,12 +944,27 @@ JobSpec job(int sys, int prec, void* in, void* out, int gkind, double scale) {
 
 int build_programs(scft_ctx* c) {
   const int ns = (int)c->sys.size();
-  std::vector<JobSpec> smear, prop, prop_q, prop_qd, fwd, inv, noise;
+  std::vector<JobSpec> smear, prop, prop_q, prop_qd, fwd, inv, noise, graft_q, graft_qd;
   for (int i = 0; i < ns; i++) {
     SysHost& s = c->sys[i];
     SysDev& d = s.dev;
     smear.push_back(job(i, 0, d.wa, d.wa, 2, 1.0));
     smear.push_back(job(i, 0, d.wb, d.wb, 2, 1.0));
+    if (s.grafted) {
+      smear.push_back(job(i, 0, d.wg, d.wg, 2, 1.0));
+      // graft chains, code/grafted_nanoparticle.cpp:29-38, 60-71:
+      // q[j] = exp(-smwg) IFFT[bond FFT q[j-1]], j = 1..Ng-1
+      JobSpec g = job(i, c->prec, d.qg, d.qg, 1, 1.0);
+      g.in_step = g.out_step = s.M;
+      g.in_soff = -1;
+      g.out_soff = 0;
+      g.post_a = g.post_b = d.expG;
+      g.post_switch = 0;
+      g.steps_active = s.p.Ng;
+      graft_q.push_back(g);
+      g.in = g.out = d.qgd;
+      graft_qd.push_back(g);
+    }
     // propagator step: q[step-1] -> q[step], exp(-w) on the last store
     JobSpec q = job(i, c->prec, d.q, d.q, 1, 1.0);
     q.in_step = q.out_step = s.M;
@@ -888,6 +989,18 @@ int build_programs(scft_ctx* c) {
     for (void* a : {(void*)d.rhoA, (void*)d.rhoB, (void*)d.wpl, (void*)d.wabm, (void*)d.wabp})
       fwd.push_back(job(i, 0, a, a, 0, inv_m));
     for (void* a : {(void*)d.wpl, (void*)d.wabm, (void*)d.wabp}) inv.push_back(job(i, 0, a, a, 0, 1.0));
+    // graft density and the pairs with chi != 0 (code/1s_update.cpp:27-52, 331-349)
+    if (s.grafted) fwd.push_back(job(i, 0, d.rhog, d.rhog, 0, inv_m));
+    if (s.pairs && s.chi_bc != 0.0)
+      for (void* a : {(void*)d.wbcm, (void*)d.wbcp}) {
+        fwd.push_back(job(i, 0, a, a, 0, inv_m));
+        inv.push_back(job(i, 0, a, a, 0, 1.0));
+      }
+    if (s.pairs && s.chi_ac != 0.0)
+      for (void* a : {(void*)d.wacm, (void*)d.wacp}) {
+        fwd.push_back(job(i, 0, a, a, 0, inv_m));
+        inv.push_back(job(i, 0, a, a, 0, 1.0));
+      }
     if (d.etap) {
       noise.push_back(job(i, 0, d.etap, d.etap, 0, inv_m));
       noise.push_back(job(i, 0, (char*)d.etap + (size_t)s.M * 16, (char*)d.etap + (size_t)s.M * 16, 0, inv_m));
@@ -897,6 +1010,13 @@ int build_programs(scft_ctx* c) {
   int rc;
   if ((rc = build_program(c, scft::MODE_CONV, smear, &c->prog_smear))) return rc;
   if (!c->ckpt && (rc = build_program(c, scft::MODE_CONV, prop, &c->prog_prop))) return rc;
+  c->any_graft = !graft_q.empty();
+  for (auto& s : c->sys) c->any_pairs = c->any_pairs || s.pairs;
+  if (c->any_graft) {
+    for (auto& s : c->sys) c->maxNg = std::max(c->maxNg, s.grafted ? s.p.Ng : 0);
+    if ((rc = build_program(c, scft::MODE_CONV, graft_q, &c->prog_graft_q))) return rc;
+    if ((rc = build_program(c, scft::MODE_CONV, graft_qd, &c->prog_graft_qd))) return rc;
+  }
   if (c->ckpt) {
     // chain steps inside the windows: slice u from slice u - 1
     std::vector<JobSpec> ckq, ckqd;
@@ -945,6 +1065,23 @@ int build_programs(scft_ctx* c) {
 // ---------------------------------------------------------------------------
 // stages
 // ---------------------------------------------------------------------------
+// Chains grafted to the plates, code/calc_poly_density.cpp:131-133, 155-160:
+// free-end propagator, complementary propagator from the graft sites,
+// graft density and the log-average Qga_exp
+int stage_graft(scft_ctx* c) {
+  if (!c->any_graft) return SCFT_OK;
+  const int ns = (int)c->sys.size();
+  int rc;
+  for (int step = 1; step < c->maxNg; step++)
+    if ((rc = run_program(c, c->prog_graft_q, step))) return rc;
+  HIPCK(scft::launch_graft_init(c->prec, c->d_sys, ns, c->maxM, c->st));
+  for (int step = 1; step < c->maxNg; step++)
+    if ((rc = run_program(c, c->prog_graft_qd, step))) return rc;
+  HIPCK(scft::launch_graft_density(c->prec, c->d_sys, ns, c->maxM, c->st));
+  HIPCK(scft::launch_reduce(scft::RED_QGA, c->prec, c->d_sys, ns, c->nblk, c->st));
+  return SCFT_OK;
+}
+
 // Checkpointed propagators and densities.  Array ids of launch_ck_copy: 0 q
 // window, 1 q-dagger window, 2 checkpoints.  Slice u of a window holds the
 // chain at contour index first + u - 1 of its segment.
@@ -1006,7 +1143,10 @@ int stage_poly_density(scft_ctx* c) {
   if (rc) return rc;
   if (c->prec == 1) HIPCK(scft::launch_reduce(scft::RED_SHIFT, 0, c->d_sys, ns, c->nblk, c->st));
   HIPCK(scft::launch_exp_prop(c->prec, c->d_sys, ns, c->maxM, c->st));
-  if (c->ckpt) return stage_ck_density(c);
+  if (c->ckpt) {
+    if ((rc = stage_ck_density(c))) return rc;
+    return stage_graft(c);
+  }
   if (c->split_chains) {
     HIPCK(hipEventRecord(c->ev_fork, c->st));
     HIPCK(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
@@ -1024,7 +1164,7 @@ int stage_poly_density(scft_ctx* c) {
   }
   HIPCK(scft::launch_reduce(scft::RED_Q, c->prec, c->d_sys, ns, c->nblk, c->st));
   HIPCK(scft::launch_density(c->prec, c->d_sys, ns, c->maxM, c->st));
-  return SCFT_OK;
+  return stage_graft(c);
 }
 
 int stage_update(scft_ctx* c, int scheme) {
@@ -1043,6 +1183,75 @@ int stage_update(scft_ctx* c, int scheme) {
   return stage_poly_density(c);
 }
 
+// Graft sites of system si, code/initialize.cpp:160-196: the sum over plates
+// of |grad rho| of the plate shape inflated by 1.05 in length and radius
+// (spectral derivative, field_gradient at code/array_utils.cpp:565-579),
+// normalised to unit integral.
+int graft_sites(scft_ctx* c, int si) {
+  SysHost& s = c->sys[si];
+  const int M = s.M;
+  const size_t m16 = (size_t)M * 16;
+  std::vector<void*> tmp;
+  auto scratch_alloc = [&](void** p) -> int {
+    if (hipMalloc(p, m16) != hipSuccess) return fail(SCFT_ENOMEM, "no memory for the graft sites");
+    tmp.push_back(*p);
+    return SCFT_OK;
+  };
+  void *spec = nullptr, *g2 = nullptr;
+  int rc = scratch_alloc(&spec);
+  if (rc == SCFT_OK) rc = scratch_alloc(&g2);
+  std::vector<cd> shape(M);
+  for (int np = 0; np < std::min(s.p.n_exp_nr, 2) && rc == SCFT_OK; np++) {
+    for (int i = 0; i < M; i++) {
+      const int nn[3] = {i % s.nx, (i / s.nx) % s.ny, i / (s.nx * s.ny)};
+      shape[i] = plate_rho(s, np, nn, 1.05);
+    }
+    if (hipMemcpy(c->scratch[si], shape.data(), m16, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemsetAsync(g2, 0, m16, c->st) != hipSuccess) {
+      rc = fail(SCFT_EDEVICE, "copy failed");
+      break;
+    }
+    if ((rc = run_program(c, c->prog_fft_fwd[si], 0))) break;
+    if (hipMemcpyAsync(spec, c->scratch[si], m16, hipMemcpyDeviceToDevice, c->st) != hipSuccess) {
+      rc = fail(SCFT_EDEVICE, "copy failed");
+      break;
+    }
+    for (int d = 0; d < s.p.dim && rc == SCFT_OK; d++) {
+      if (scft::launch_ik_mult(c->d_sys + si, M, (const double2*)spec, (double2*)c->scratch[si], d, c->st) != hipSuccess)
+        rc = fail(SCFT_EDEVICE, "graft-site kernel failed");
+      if (rc == SCFT_OK) rc = run_program(c, c->prog_fft_inv[si], 0);
+      if (rc == SCFT_OK &&
+          scft::launch_sites_accum(M, (const double2*)c->scratch[si], (double2*)g2, 0, c->st) != hipSuccess)
+        rc = fail(SCFT_EDEVICE, "graft-site kernel failed");
+    }
+    if (rc == SCFT_OK && scft::launch_sites_accum(M, (const double2*)g2, s.dev.expl, 1, c->st) != hipSuccess)
+      rc = fail(SCFT_EDEVICE, "graft-site kernel failed");
+  }
+  // normalise on the host (set-up only)
+  if (rc == SCFT_OK) {
+    std::vector<cd> e(M);
+    if (hipMemcpyAsync(e.data(), s.dev.expl, m16, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess) {
+      rc = fail(SCFT_EDEVICE, "copy failed");
+    } else {
+      cd total = 0.0;
+      for (int i = 0; i < M; i++) total += e[i];
+      total *= s.dV;
+      cd after = 0.0;
+      for (int i = 0; i < M; i++) {
+        e[i] /= total;
+        after += e[i];
+      }
+      s.expl_total = after * s.dV;
+      if (hipMemcpy(s.dev.expl, e.data(), m16, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(SCFT_EDEVICE, "copy failed");
+    }
+  }
+  (void)hipStreamSynchronize(c->st);
+  for (void* p : tmp) (void)hipFree(p);
+  return rc;
+}
+
 // one iteration of the reference main loop (code/simulate.cpp:86-92):
 // update, then the complex-Langevin averages
 int stage_iteration(scft_ctx* c, int scheme) {
@@ -1050,6 +1259,13 @@ int stage_iteration(scft_ctx* c, int scheme) {
   if (rc) return rc;
   if (c->any_cl) HIPCK(scft::launch_cl_accumulate(c->d_sys, (int)c->sys.size(), c->maxM, c->st));
   return SCFT_OK;
+}
+
+// Qga_exp = integ(expl_grafts log q_g[Ng-1]) from its reduction; with the
+// gauge shift c the free-end propagator carries exp(Ng c)
+cd qga(const SysHost& s, const double2* red) {
+  const cd sum = cd(red[scft::RED_QGA].x, red[scft::RED_QGA].y) * s.dV;
+  return sum - double(s.p.Ng) * red[scft::RED_SHIFT].y * s.expl_total;
 }
 
 int check_ctx(scft_ctx* c, int sys) {
@@ -1162,6 +1378,8 @@ int scft_create(const scft_params* params, int n_systems, int device, int precis
         hipMemcpyAsync(s.dev.exp_nrH, c->scratch[i], (size_t)s.M * 16, hipMemcpyDeviceToDevice, c->st) != hipSuccess)
       rc = fail(SCFT_EDEVICE, "copy failed");
   }
+  for (int i = 0; i < n_systems && rc == SCFT_OK; i++)
+    if (c->sys[i].has_sites) rc = graft_sites(c, i);
   if (rc == SCFT_OK && hipStreamSynchronize(c->st) != hipSuccess) rc = fail(SCFT_EDEVICE, "set-up failed on the device");
   if (rc == SCFT_OK && !getenv("SCFT_NO_TUNE")) {
     // tuning runs on zeroed propagators and scratch fields (recomputed later)
@@ -1344,6 +1562,69 @@ int scft_get_averages(scft_ctx* c, int sys, double* avg_a, double* avg_b, double
   return SCFT_OK;
 }
 
+namespace {
+// Named complex128 arrays of M values (the reference's globals,
+// code/globals.h:76-123); nullptr when the system does not hold the array.
+void* named_array(scft_ctx* c, SysHost& s, const char* name, bool writable) {
+  const std::string n(name);
+  SysDev& d = s.dev;
+  if (n == "wpl") return d.wpl;
+  if (n == "wabp") return d.wabp;
+  if (n == "wabm") return d.wabm;
+  if (n == "wbcp") return d.wbcp;
+  if (n == "wbcm") return d.wbcm;
+  if (n == "wacp") return d.wacp;
+  if (n == "wacm") return d.wacm;
+  if (writable) return nullptr;
+  if (n == "rhoda") return d.rhoA;
+  if (n == "rhodb") return d.rhoB;
+  if (n == "smwa") return d.wa;
+  if (n == "smwb") return d.wb;
+  if (n == "rho_exp_nr") return d.rho_np;
+  if (n == "smwg") return d.wg;
+  if (n == "expl_grafts") return d.expl;
+  if (n == "rhog_exp") return d.rhog;
+  if (n.rfind("qg:", 0) == 0 && d.qg && c->prec == 0) {
+    const int j = std::atoi(n.c_str() + 3);
+    if (j >= 0 && j < s.p.Ng) return (char*)d.qg + (size_t)j * s.M * 16;
+  }
+  return nullptr;
+}
+}  // namespace
+
+int scft_get_array(scft_ctx* c, int sys, const char* name, double* out) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  if (!name || !out) return fail(SCFT_EINVAL, "null argument");
+  SysHost& s = c->sys[sys];
+  const void* p = named_array(c, s, name, false);
+  if (!p) return fail(SCFT_EINVAL, std::string("system holds no array named ") + name);
+  return copy_out(c, out, p, (size_t)s.M * 16);
+}
+
+int scft_set_array(scft_ctx* c, int sys, const char* name, const double* in) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  if (!name || !in) return fail(SCFT_EINVAL, "null argument");
+  SysHost& s = c->sys[sys];
+  void* p = named_array(c, s, name, true);
+  if (!p) return fail(SCFT_EINVAL, std::string("system holds no settable field named ") + name);
+  return copy_in(c, p, in, (size_t)s.M * 16);
+}
+
+int scft_get_Qga(scft_ctx* c, int sys, double* Qga) {
+  int rc = check_ctx(c, sys);
+  if (rc) return rc;
+  SysHost& s = c->sys[sys];
+  if (!s.grafted) return fail(SCFT_EINVAL, "system has no grafted chains");
+  double2 red[scft::RED_NSLOTS];
+  if ((rc = copy_out(c, red, s.dev.red, sizeof red))) return rc;
+  const cd q = qga(s, red);
+  Qga[0] = q.real();
+  Qga[1] = q.imag();
+  return SCFT_OK;
+}
+
 int scft_set_plates(scft_ctx* c, int sys, const double* centres, const double* u) {
   int rc = check_ctx(c, sys);
   if (rc) return rc;
@@ -1385,8 +1666,17 @@ int scft_set_plates(scft_ctx* c, int sys, const double* centres, const double* u
     // X(k) and X2(k) carry phiD as a factor
     HIPCK(scft::launch_scale_real(s.dev.Xk, s.M, inf.phiD / phiD_old, c->st));
     HIPCK(scft::launch_scale_real(s.dev.X2k, s.M, inf.phiD / phiD_old, c->st));
+    if (s.pairs) {
+      HIPCK(scft::launch_scale_real(s.dev.XAk, s.M, inf.phiD / phiD_old, c->st));
+      HIPCK(scft::launch_scale_real(s.dev.XBk, s.M, inf.phiD / phiD_old, c->st));
+    }
   }
   HIPCK(hipMemcpyAsync(row, &s.dev, sizeof(SysDev), hipMemcpyHostToDevice, c->st));
+  if (s.has_sites) {
+    // the graft sites follow the plates
+    HIPCK(hipMemsetAsync(s.dev.expl, 0, (size_t)s.M * 16, c->st));
+    if ((rc = graft_sites(c, sys))) return rc;
+  }
   HIPCK(hipStreamSynchronize(c->st));
   return SCFT_OK;
 }
@@ -1443,6 +1733,7 @@ int scft_update_1s(scft_ctx* c) {
 int scft_update_euler(scft_ctx* c) {
   int rc = check_ctx(c, 0);
   if (rc) return rc;
+  if (c->any_pairs) return fail(SCFT_EUNSUPPORTED, "the Euler update has no BC/AC field pairs");
   rc = stage_update(c, 0);
   if (rc) return rc;
   HIPCK(hipStreamSynchronize(c->st));
@@ -1465,6 +1756,8 @@ int scft_calc_H(scft_ctx* c, double* H) {
     cd logQ = std::log(Qs) - double(s.N) * red[scft::RED_SHIFT].y;
     cd h = field;
     if (s.info.nD > 0.0) h += -s.info.nD * logQ;
+    // chains grafted to the plates, code/calc_h.cpp:31-35
+    if (s.grafted) h += -double(s.p.n_exp_nr) * s.info.ng_per_np * qga(s, red).real();
     H[2 * i] = h.real();
     H[2 * i + 1] = h.imag();
     if (std::isnan(h.real()) || std::isnan(h.imag())) nan = 1;
@@ -1595,6 +1888,8 @@ int scft_describe(scft_ctx* c, char* buf, int len) {
                slices, slices * slice / 1048576.0, c->ck_K, c->ck_ne);
     else
       snprintf(tmp, sizeof tmp, "propagator_storage: %d slices, %.1f MiB\n", slices, slices * slice / 1048576.0);
+    o += tmp;
+    snprintf(tmp, sizeof tmp, "device_bytes: %zu\n", c->dev_bytes);
     o += tmp;
   }
   snprintf(buf, len, "%s", o.c_str());

@@ -89,6 +89,20 @@ struct SysDev {
   double len_nr, R_nr, xi_nr;
   PlateDev plate[2];
   double2 *grad_partials;   // 8 complex per block: plate derivatives of H
+  // BC / AC field pairs (code/globals.h:76-90; null when the pair is off)
+  // and the chains grafted to the plates (code/grafted_nanoparticle.cpp)
+  double2 *wbcp, *wbcm, *wacp, *wacm;
+  int neg_bc, neg_ac;
+  double chi_bc, chi_ac;    // |chi|
+  int grafted, Ng, n_exp;
+  double ng_per_np;
+  double2 *wg;              // assembled graft field, smeared in place -> smwg
+  double2 *expl;            // graft sites, unit integral
+  double2 *rhog;            // graft density rhog_exp
+  void *qg, *qgd;           // Ng * M graft propagators, propagator precision
+  void *expG;               // exp(-(smwg - c))
+  double *XGk;              // phiG rho0 Ng h^2 gc
+  double *XAk, *XBk;        // phiD rho0 N h^2 gaa, gbb
 };
 
 enum {
@@ -97,7 +111,8 @@ enum {
   RED_SHIFT = 2,   // sum of Re(fA smwa + (1 - fA) smwb); .y = shift c
   RED_RHO = 3,     // sum of rhoA + rhoB (segment count check)
   RED_VEXP = 4,    // sum of the plate density (volume bookkeeping)
-  RED_NSLOTS = 5
+  RED_QGA = 5,     // sum of expl_grafts log q_g[Ng-1] (shifted gauge)
+  RED_NSLOTS = 6
 };
 
 }  // namespace scft

@@ -210,8 +210,39 @@ __global__ __launch_bounds__(256) void k_assemble(const SysDev* __restrict__ sys
     a = make_double2(iwp.x + (im.x - p.x), iwp.y + (im.y - p.y));
     b = make_double2(iwp.x + (-im.x - p.x), iwp.y + (-im.y - p.y));
   }
+  // BC / AC pairs and the graft field, code/calc_poly_density.cpp:44-53:
+  //   wa += i wAC+ - wAC-  [negative chi: i wAC- - wAC+]
+  //   wb += i wBC+ - wBC-  [negative chi: i wBC- - wBC+]
+  //   wg = i w+ + (i wAC+ + wAC-) + (i wBC+ + wBC-)
+  //        [negative chi: -i wAC- - wAC+, -i wBC- - wBC+]
+  double2 g = iwp;
+  if (s.wacp) {
+    const double2 cp = s.wacp[i], cm = s.wacm[i];
+    if (!s.neg_ac) {
+      const double2 icp = mul_i(cp);
+      a = dadd(a, make_double2(icp.x - cm.x, icp.y - cm.y));
+      g = dadd(g, make_double2(icp.x + cm.x, icp.y + cm.y));
+    } else {
+      const double2 icm = mul_i(cm);
+      a = dadd(a, make_double2(icm.x - cp.x, icm.y - cp.y));
+      g = dadd(g, make_double2(-icm.x - cp.x, -icm.y - cp.y));
+    }
+  }
+  if (s.wbcp) {
+    const double2 cp = s.wbcp[i], cm = s.wbcm[i];
+    if (!s.neg_bc) {
+      const double2 icp = mul_i(cp);
+      b = dadd(b, make_double2(icp.x - cm.x, icp.y - cm.y));
+      g = dadd(g, make_double2(icp.x + cm.x, icp.y + cm.y));
+    } else {
+      const double2 icm = mul_i(cm);
+      b = dadd(b, make_double2(icm.x - cp.x, icm.y - cp.y));
+      g = dadd(g, make_double2(-icm.x - cp.x, -icm.y - cp.y));
+    }
+  }
   s.wa[i] = a;
   s.wb[i] = b;
+  if (s.wg) s.wg[i] = g;
 }
 
 // exp(-(smw - c)) into the propagator precision, and the chain ends
@@ -231,6 +262,84 @@ __global__ __launch_bounds__(256) void k_exp_prop(const SysDev* __restrict__ sys
   reinterpret_cast<cpx<R>*>(s.expB)[i] = cb;
   reinterpret_cast<cpx<R>*>(s.q)[i] = (s.Na != 0) ? ca : cb;
   if (s.qd != s.q) reinterpret_cast<cpx<R>*>(s.qd)[i] = (s.Na != s.N) ? cb : ca;
+  if (s.grafted) {
+    // free-end graft propagator q_g[0] = exp(-smwg),
+    // code/grafted_nanoparticle.cpp:24-26
+    const double2 g = s.wg[i];
+    const double2 eg = dexp(make_double2(-(g.x - c), -g.y));
+    const cpx<R> cg = mk(R(eg.x), R(eg.y));
+    reinterpret_cast<cpx<R>*>(s.expG)[i] = cg;
+    reinterpret_cast<cpx<R>*>(s.qg)[i] = cg;
+  }
+}
+
+// complementary graft propagator start, code/grafted_nanoparticle.cpp:55-57:
+// q_g-dagger[0] = expl_grafts exp(-smwg) / q_g[Ng-1]
+template <typename R>
+__global__ __launch_bounds__(256) void k_graft_init(const SysDev* __restrict__ sys) {
+  const SysDev& s = sys[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (!s.grafted || i >= s.M) return;
+  const cpx<R> e = reinterpret_cast<const cpx<R>*>(s.expG)[i];
+  const cpx<R> ql = reinterpret_cast<const cpx<R>*>(s.qg)[(long)(s.Ng - 1) * s.M + i];
+  const double2 num = dmul(s.expl[i], make_double2((double)e.x, (double)e.y));
+  const double2 v = ddiv(num, make_double2((double)ql.x, (double)ql.y));
+  reinterpret_cast<cpx<R>*>(s.qgd)[i] = mk(R(v.x), R(v.y));
+}
+
+// graft density, code/grafted_nanoparticle.cpp:73-80: rhog_exp =
+// ng_per_np n_plates exp(smwg) sum_j q_g[j] q_g-dagger[Ng-1-j] (FP64
+// accumulation; with the gauge shift the product carries exp(c))
+template <typename R>
+__global__ __launch_bounds__(256) void k_graft_density(const SysDev* __restrict__ sys) {
+  const SysDev& s = sys[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (!s.grafted || i >= s.M) return;
+  const cpx<R>* q = reinterpret_cast<const cpx<R>*>(s.qg);
+  const cpx<R>* qd = reinterpret_cast<const cpx<R>*>(s.qgd);
+  const long M = s.M;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int j = 0; j < s.Ng; j++) {
+    const cpx<R> a = q[(long)j * M + i];
+    const cpx<R> b = qd[(long)(s.Ng - 1 - j) * M + i];
+    acc = dadd(acc, make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x));
+  }
+  const double c = s.red[RED_SHIFT].y;
+  const double2 g = s.wg[i];
+  const double2 e = dexp(make_double2(g.x - c, g.y));
+  s.rhog[i] = dscale(dmul(acc, e), s.ng_per_np * (double)s.n_exp);
+}
+
+// Graft sites (code/initialize.cpp:160-196, field_gradient at
+// code/array_utils.cpp:565-579): out = i k_d in (spectrum), k_d the aliased
+// wavenumber (0 at the Nyquist mode of an even axis)
+__global__ __launch_bounds__(256) void k_ik_mult(const SysDev* __restrict__ sys, const double2* __restrict__ in,
+                                                 double2* __restrict__ out, int d) {
+  const SysDev& s = sys[0];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.M) return;
+  const int n[3] = {(int)(i % s.nx), (int)((i / s.nx) % s.ny), (int)(i / ((long)s.nx * s.ny))};
+  const int Ns[3] = {s.nx, s.ny, s.nz};
+  double k;
+  if (Ns[d] % 2 == 0 && n[d] == Ns[d] / 2) k = 0.0;
+  else if ((double)n[d] < (double)Ns[d] / 2.0) k = 2.0 * 3.14159265358979323846 * n[d] / s.Lb[d];
+  else k = 2.0 * 3.14159265358979323846 * (n[d] - Ns[d]) / s.Lb[d];
+  out[i] = mul_i(dscale(in[i], k));
+}
+// mode 0: acc += a * a;  mode 1: acc2 += sqrt(acc) (principal branch)
+__global__ __launch_bounds__(256) void k_sites_accum(long M, const double2* __restrict__ a, double2* __restrict__ acc,
+                                                     int mode) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= M) return;
+  const double2 v = a[i];
+  if (mode == 0) {
+    acc[i] = dadd(acc[i], dmul(v, v));
+  } else {
+    const double r = sqrt(v.x * v.x + v.y * v.y);
+    const double re = sqrt(0.5 * (r + v.x));
+    const double im = copysign(sqrt(0.5 * fmax(r - v.x, 0.0)), v.y);
+    acc[i] = dadd(acc[i], make_double2(re, im));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -255,6 +364,16 @@ __device__ __forceinline__ double2 red_term(const SysDev& s, long i) {
     const double2 c = make_double2(s.rho0 * (1.0 - s.rho_np[i].x), -s.rho0 * s.rho_np[i].y);
     const double2 icw = mul_i(dmul(c, w));
     t = make_double2(t.x - icw.x, t.y - icw.y);
+    // wbc_part, wac_part, wab_part (code/calc_h.cpp:89-132), in the order
+    // of the sum at :13
+    if (s.wbcp && s.chi_bc != 0.0) {
+      const double2 p = s.wbcp[i], m = s.wbcm[i];
+      t = dadd(t, dscale(dadd(dmul(p, p), dmul(m, m)), s.rho0 / s.chi_bc * (s.neg_bc ? -1.0 : 1.0)));
+    }
+    if (s.wacp && s.chi_ac != 0.0) {
+      const double2 p = s.wacp[i], m = s.wacm[i];
+      t = dadd(t, dscale(dadd(dmul(p, p), dmul(m, m)), s.rho0 / s.chi_ac * (s.neg_ac ? -1.0 : 1.0)));
+    }
     if (s.chi_ab != 0.0) {
       const double2 p = s.wabp[i], m = s.wabm[i];
       const double2 pp = dmul(p, p), mm = dmul(m, m);
@@ -266,6 +385,16 @@ __device__ __forceinline__ double2 red_term(const SysDev& s, long i) {
     return make_double2(s.fA * s.wa[i].x + (1.0 - s.fA) * s.wb[i].x, 0.0);
   } else if constexpr (WHAT == RED_VEXP) {
     return s.rho_np[i];
+  } else if constexpr (WHAT == RED_QGA) {
+    // code/grafted_nanoparticle.cpp:83-84: expl_grafts log q_g[Ng-1] where
+    // Re expl_grafts > 0
+    if (!s.grafted) return make_double2(0.0, 0.0);
+    const double2 e = s.expl[i];
+    if (e.x <= 0.0) return make_double2(0.0, 0.0);
+    const cpx<R> v = ld<R>(s.qg, (long)(s.Ng - 1) * s.M + i);
+    const double vx = v.x, vy = v.y;
+    const double2 lg = make_double2(0.5 * log(vx * vx + vy * vy), atan2(vy, vx));
+    return dmul(e, lg);
   } else {
     return dadd(s.rhoA[i], s.rhoB[i]);
   }
@@ -413,11 +542,14 @@ __global__ __launch_bounds__(256) void k_update_1s(const SysDev* __restrict__ sy
   const double2 rsum = dadd(rA, rB);
   const double2 enr = s.exp_nrH[i];
   const double evar = (i == 0) ? 1.0 : 0.0;
+  const double2 rg = s.grafted ? s.rhog[i] : make_double2(0.0, 0.0);  // k-space
   {
-    // F = i rho0 (exp_nrH - evar) + i h (rhoA + rhoB)
-    const double2 in = make_double2(rho0 * (enr.x - evar) + h * rsum.x, rho0 * enr.y + h * rsum.y);
+    // F = i rho0 (exp_nrH - evar) + i h (rhoA + rhog + rhoB)
+    const double2 all = dadd(dadd(rA, rg), rB);
+    const double2 in = make_double2(rho0 * (enr.x - evar) + h * all.x, rho0 * enr.y + h * all.y);
     const double2 F = mul_i(in);
-    const double A = (s.kappa <= 0.0 ? 0.0 : rho0 / s.kappa) + s.Xk[i];
+    double A = (s.kappa <= 0.0 ? 0.0 : rho0 / s.kappa) + s.Xk[i];
+    if (s.grafted) A += s.XGk[i];
     const double B = s.kappa > 0.0 ? A - rho0 / s.kappa : 0.0;
     const double2 w = s.wpl[i];
     double2 num = make_double2(w.x - li * (F.x - B * w.x), w.y - li * (F.y - B * w.y));
@@ -454,6 +586,48 @@ __global__ __launch_bounds__(256) void k_update_1s(const SysDev* __restrict__ sy
   } else {
     s.wabp[i] = make_double2(0.0, 0.0);
     s.wabm[i] = make_double2(0.0, 0.0);
+  }
+  // BC pair (rho = rhoB, g = gbb; code/1s_update.cpp:162-247) and AC pair
+  // (rho = rhoA, g = gaa; :249-329), both signs of chi
+  for (int pr = 0; pr < 2; pr++) {
+    double2* wpp = pr == 0 ? s.wbcp : s.wacp;
+    double2* wmp = pr == 0 ? s.wbcm : s.wacm;
+    if (!wpp) continue;
+    const double chi = pr == 0 ? s.chi_bc : s.chi_ac;
+    if (chi == 0.0) {
+      wpp[i] = make_double2(0.0, 0.0);
+      wmp[i] = make_double2(0.0, 0.0);
+      continue;
+    }
+    const bool neg = pr == 0 ? s.neg_bc : s.neg_ac;
+    const double2 rho = pr == 0 ? rB : rA;
+    const double G = pr == 0 ? s.XBk[i] : s.XAk[i];
+    const double GC = s.grafted ? s.XGk[i] : 0.0;
+    const double c2 = 2.0 * rho0 / chi;
+    const double2 wp = wpp[i], wm = wmp[i];
+    if (!neg) {
+      // +: F = i h (rhog + rho), A = 2 rho0/chi + G + GC, semi-implicit
+      const double2 F = mul_i(dscale(dadd(rg, rho), h));
+      const double A = c2 + G + GC;
+      const double B = A - c2;
+      wpp[i] = dscale(make_double2(wp.x - li * (F.x - B * wp.x), wp.y - li * (F.y - B * wp.y)),
+                      1.0 / (1.0 + li * A));
+      // -: F = h (rhog - rho) (+ i rho0 exp_nrH for A-like plates), explicit
+      double2 Fm = dscale(make_double2(rg.x - rho.x, rg.y - rho.y), h);
+      if (s.np_chem == 1) Fm = dadd(Fm, mul_i(dscale(enr, rho0)));
+      wmp[i] = dscale(make_double2(wm.x - lr * Fm.x, wm.y - lr * Fm.y), 1.0 / (1.0 + lr * c2));
+    } else {
+      // +: F = -h (rho + rhog), explicit
+      const double2 F = dscale(dadd(rho, rg), -h);
+      wpp[i] = dscale(make_double2(wp.x - lr * F.x, wp.y - lr * F.y), 1.0 / (1.0 + lr * c2));
+      // -: F = -i h (rhog - rho) (- rho0 exp_nrH), A = 2 rho0/chi + G - GC
+      double2 Fm = mul_i(dscale(make_double2(rg.x - rho.x, rg.y - rho.y), -h));
+      if (s.np_chem == 1) Fm = make_double2(Fm.x - rho0 * enr.x, Fm.y - rho0 * enr.y);
+      const double A = c2 + G - GC;
+      const double B = A - c2;
+      wmp[i] = dscale(make_double2(wm.x - li * (Fm.x - B * wm.x), wm.y - li * (Fm.y - B * wm.y)),
+                      1.0 / (1.0 + li * A));
+    }
   }
 }
 
@@ -806,6 +980,28 @@ hipError_t launch_exp_prop(int prec, const SysDev* s, int nsys, int maxM, hipStr
   return hipGetLastError();
 }
 
+hipError_t launch_graft_init(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st) {
+  if (prec == 0) hipLaunchKernelGGL(k_graft_init<double>, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  else hipLaunchKernelGGL(k_graft_init<float>, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_graft_density(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st) {
+  if (prec == 0) hipLaunchKernelGGL(k_graft_density<double>, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  else hipLaunchKernelGGL(k_graft_density<float>, pgrid(maxM, nsys), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_ik_mult(const SysDev* one, long M, const double2* in, double2* out, int d, hipStream_t st) {
+  hipLaunchKernelGGL(k_ik_mult, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st, one, in, out, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_sites_accum(long M, const double2* a, double2* acc, int mode, hipStream_t st) {
+  hipLaunchKernelGGL(k_sites_accum, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st, M, a, acc, mode);
+  return hipGetLastError();
+}
+
 hipError_t launch_density(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st) {
   if (prec == 0) hipLaunchKernelGGL(k_density<double>, pgrid(maxM, nsys), dim3(256), 0, st, s);
   else hipLaunchKernelGGL(k_density<float>, pgrid(maxM, nsys), dim3(256), 0, st, s);
@@ -884,6 +1080,11 @@ hipError_t launch_reduce(int what, int prec, const SysDev* s, int nsys, int nblk
     case RED_VEXP:
       hipLaunchKernelGGL((k_reduce1<RED_VEXP, double>), g1, dim3(256), 0, st, s);
       hipLaunchKernelGGL(k_reduce2<RED_VEXP>, g2, dim3(256), 0, st, s);
+      break;
+    case RED_QGA:
+      if (prec == 0) hipLaunchKernelGGL((k_reduce1<RED_QGA, double>), g1, dim3(256), 0, st, s);
+      else hipLaunchKernelGGL((k_reduce1<RED_QGA, float>), g1, dim3(256), 0, st, s);
+      hipLaunchKernelGGL(k_reduce2<RED_QGA>, g2, dim3(256), 0, st, s);
       break;
     default:
       hipLaunchKernelGGL((k_reduce1<RED_RHO, double>), g1, dim3(256), 0, st, s);

@@ -334,7 +334,8 @@ def write_input(p: ScftParams, path: str) -> None:
           f"{p.Nu}               Nu, number of points per pi for spherical integration",
           f"{_num(p.np_frac)}              np_frac, volume fraction of field-based nps",
           f"{_num(p.L_nr)} {_num(p.R_nr)} {_num(p.xi_nr)}      L_nr, R_nr, xi_nr for all nps"]
-    for j in range(min(p.n_exp_nr, 2)):
+    # read_input always reads two plate blocks (code/read_input.cpp:205)
+    for j in range(2):
         L.append(f"{vec(p.exp_nr_c[j])}   Nanorod center x, y, z for np {j + 1}")
         L.append(f"{vec(p.exp_nr_u[j])}  Nanorod orientation vector x, y, z for np {j + 1}")
     with open(path, "w") as f:
@@ -415,6 +416,9 @@ def load_library(path: str | None = None):
         "scft_fft": (ctypes.c_int, [vp, ctypes.c_int, dp, ctypes.c_int]),
         "scft_get_noise": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp]),
         "scft_get_averages": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp]),
+        "scft_get_array": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_char_p, dp]),
+        "scft_set_array": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_char_p, dp]),
+        "scft_get_Qga": (ctypes.c_int, [vp, ctypes.c_int, dp]),
         "scft_set_plates": (ctypes.c_int, [vp, ctypes.c_int, dp, dp]),
         "scft_grad_plates": (ctypes.c_int, [vp, ctypes.c_int, dp, dp]),
         "scft_calc_poly_density": (ctypes.c_int, [vp]),
@@ -565,6 +569,23 @@ class Context:
         n = np.zeros(1)
         self._check(self.lib.scft_get_averages(self.h, sys, _dptr(a), _dptr(b), _dptr(n)), "scft_get_averages")
         return a, b, float(n[0])
+
+    def array(self, name: str, sys=0):
+        """A named complex array of M values (fields, densities, graft sites)."""
+        out = np.empty(self.M(sys), np.complex128)
+        self._check(self.lib.scft_get_array(self.h, sys, name.encode(), _dptr(out)), "scft_get_array")
+        return out
+
+    def set_array(self, name: str, a, sys=0):
+        x = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+        if x.size != self.M(sys):
+            raise ValueError(f"{name}: expected {self.M(sys)} values, got {x.size}")
+        self._check(self.lib.scft_set_array(self.h, sys, name.encode(), _dptr(x)), "scft_set_array")
+
+    def get_Qga(self, sys=0) -> complex:
+        out = np.empty(2)
+        self._check(self.lib.scft_get_Qga(self.h, sys, _dptr(out)), "scft_get_Qga")
+        return complex(out[0], out[1])
 
     def fft(self, a, direction: int, sys=0):
         x = np.array(a, dtype=np.complex128, copy=True)

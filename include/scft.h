@@ -141,12 +141,30 @@ int scft_get_cavity(scft_ctx* ctx, int sys, double* rho);
 /* Single-chain partition function Qd of the last density evaluation. */
 int scft_get_Q(scft_ctx* ctx, int sys, double* Q_re_im);
 
+/* Named arrays of M interleaved complex values (the reference's globals,
+ * code/globals.h:76-123, and the .res / .bin files of code/io_utils.cpp).
+ * Settable fields: "wpl", "wabp", "wabm" and, for systems with BC/AC field
+ * pairs (grafted chains, chi_BC or chi_AC nonzero), "wbcp", "wbcm", "wacp",
+ * "wacm" (the .res overrides of
+ * read_resume_files, code/io_utils.cpp:183-240).  Readable in addition:
+ * "rhoda", "rhodb", "smwa", "smwb", "rho_exp_nr", with rod plates
+ * "expl_grafts" (graft sites, code/initialize.cpp:160-205, computed for rods
+ * whether or not chains are grafted) and, with grafts, "smwg", "rhog_exp"
+ * (graft density) and, in FP64, "qg:j" (free-end graft propagator at
+ * contour index j).  SCFT_EINVAL for a name the system does not hold. */
+int scft_get_array(scft_ctx* ctx, int sys, const char* name, double* out);
+int scft_set_array(scft_ctx* ctx, int sys, const char* name, const double* in);
+/* Qga_exp = integ(expl_grafts log q_g[Ng-1]) of the last density evaluation
+ * (code/calc_poly_density.cpp:155-160); SCFT_EINVAL without grafts. */
+int scft_get_Qga(scft_ctx* ctx, int sys, double* Qga_re_im);
+
 /* Move or rotate the plates of system `sys` in place: `centres` and `u` hold
  * 3 doubles per plate (fractional centres 0..1, orientation vectors,
  * normalised here), as the plate lines of bcp.input (code/read_input.cpp:
  * 205-236).  Recomputes the plate density on the device
- * (code/initialize.cpp:578-612), its transform exp_nrH and the volume
- * bookkeeping nD, phi_D (code/initialize.cpp:249-355); fields are kept. */
+ * (code/initialize.cpp:578-612), its transform exp_nrH, the volume
+ * bookkeeping nD, phi_D (code/initialize.cpp:249-355) and, with grafted
+ * chains, the graft sites; fields are kept. */
 int scft_set_plates(scft_ctx* ctx, int sys, const double* centres,
                     const double* u);
 

@@ -15,6 +15,10 @@ Outputs
   bcp_angle-45.00.input    parser fixtures (data files)
   fft_vectors.npz          numpy.fft (pocketfft) golden vectors, forward
                            e^{-i} and backward e^{+i}, unnormalised
+  log_templates.json       the line sequence of two published LOGs with every
+                           number replaced by '#' (logfmt.log_template)
+
+    python tests/golden/make_fixtures.py --templates   # log_templates.json only
 """
 import glob
 import json
@@ -23,6 +27,8 @@ import re
 import shutil
 
 import numpy as np
+
+from logfmt import log_template
 
 REF = "/root/reference"
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -46,6 +52,14 @@ def parse_log(path):
         "ng_per_np": last(r"# of grafts per np = (\S+)"),
         "V": last(r"\bV = (\S+)"),
     }
+
+
+def write_log_templates():
+    runs = {}
+    for sweep, run in (("angle-rotation", "angle-05.00"), ("particle-separation", "dist-0.300")):
+        runs[run] = log_template(open(os.path.join(REF, sweep, run, "LOG")).read())
+    with open(os.path.join(HERE, "log_templates.json"), "w") as f:
+        json.dump(runs, f, indent=1)
 
 
 def parse_hamiltonians(path):
@@ -100,8 +114,13 @@ def main():
         vec[f"a{key}"] = a
         vec[f"A{key}"] = np.fft.fftn(a)
     np.savez_compressed(os.path.join(HERE, "fft_vectors.npz"), **vec)
+    write_log_templates()
     print("wrote", len(runs), "runs and", len(vec), "fft arrays")
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+    if "--templates" in sys.argv:
+        write_log_templates()
+    else:
+        main()

@@ -239,3 +239,83 @@ def test_driver_cl_run_writes_averages(tmp_path):
     assert n == 5.0
     assert np.max(np.abs(avg - a)) <= 1e-12 * np.max(np.abs(a))
     ctx.close()
+
+
+# ------------------------------------------- grafted runs and the LOG lines
+def _log_template(text):
+    sys.path.insert(0, GOLDEN)
+    try:
+        from logfmt import log_template
+    finally:
+        sys.path.remove(GOLDEN)
+    return log_template(text)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("run,name", [("angle-05.00", "bcp_angle-45.00.input"),
+                                      ("dist-0.300", "bcp_dist-0.300.input")])
+def test_driver_log_follows_published_log(tmp_path, run, name):
+    # the published input at its published grid, stopped by the tolerance
+    # rule at iteration 26 (error_tol raised), against the line sequence of
+    # the published LOG with numbers masked
+    need_driver()
+    import json
+    want = json.load(open(os.path.join(GOLDEN, "log_templates.json")))[run]
+    p = scft_amd.read_input(os.path.join(GOLDEN, name), 2)
+    p.print_freq = 1
+    p.error_tol = 1e3
+    scft_amd.write_input(p, str(tmp_path / "bcp.input"))
+    print("scft_run LOG", run, flush=True)
+    r = subprocess.run([DRIVER, "--dim", "2", "--no-bin"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    got = _log_template(r.stdout)
+    assert got == want, "\n".join(f"{a!r} | {b!r}" for a, b in zip(got, want) if a != b)
+    rows = scft_amd.read_data_dat(str(tmp_path / "data.dat"))
+    assert rows[-1, 0] == 26
+
+
+@pytest.mark.gpu
+def test_driver_grafted_run_matches_oracle(tmp_path):
+    # 2D box with the published chemistry (grafts, chi_BC < 0), all seven
+    # start fields from .res files
+    need_driver()
+    p = scft_amd.read_input(os.path.join(GOLDEN, "bcp_dist-0.300.input"), 2)
+    p.Nda, p.Ndb, p.Ng = 10, 10, 4
+    p.Nx[0], p.Nx[1] = 48, 80
+    p.L[0], p.L[1] = 24.0, 40.0
+    p.itermax, p.print_freq, p.error_tol = 30, 5, 1e-14
+    names = ["wpl", "wabp", "wabm", "wbcp", "wbcm", "wacp", "wacm"]
+    rng = np.random.default_rng(12)
+    units = [1j, 1j, 1.0, 1.0, 1j, 1j, 1.0]   # chi_BC < 0 swaps the BC pair
+    f = [0.05 * rng.standard_normal(p.M) * u for u in units]
+    scft_amd.write_input(p, str(tmp_path / "bcp.input"))
+    for n, a in zip(names, f):
+        scft_amd.write_res(str(tmp_path / f"{n}.res"), a, p)
+    print("scft_run grafted 2D", tmp_path, flush=True)
+    r = subprocess.run([DRIVER, "--dim", "2"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    out = r.stdout
+    assert out.count("Processor 0 reading from line 1. x = [ 0.000000 0.000000 ]") == 7
+    ref = pyoracle.Oracle(scft_amd.read_input(str(tmp_path / "bcp.input"), 2))
+    for n, a in zip(names, f):
+        ref.set_array(n, a)
+    path = str(tmp_path / "oracle_data.dat")
+    ref.simulate(data_path=path)
+    got = scft_amd.read_data_dat(str(tmp_path / "data.dat"))
+    want = scft_amd.read_data_dat(path)
+    _compare_rows(got, want)
+    # -log(Qg) = -n ng_per_np Re(Qga) of the last printed iteration (30 = itermax)
+    last = [l for l in out.splitlines() if l.startswith("Iter: 30,")][0]
+    qg = float(last.split("-log(Qg)=")[1].split(",")[0])
+    ng = ref.scalar("ng_per_np")
+    assert abs(qg - (-p.n_exp_nr * ng * ref.scalar("Qga_re"))) <= 5.01e-7 + 1e-9 * abs(qg)
+    # dumps of the last print: graft density, graft sites, pair fields
+    for n, tol in (("rhog_exp_c", 1e-9), ("expl_grafts", 1e-12), ("wbcp", 1e-9), ("wacm", 1e-9)):
+        a = scft_amd.read_bin(str(tmp_path / f"{n}.p0.bin"))[4]
+        b = ref.array(n[:-2] if n.endswith("_c") else n)
+        assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(b)), n
+    for n in ("rhog", "rhog_c"):
+        assert np.all(scft_amd.read_bin(str(tmp_path / f"{n}.p0.bin"))[4] == 0)
+    assert os.path.exists(tmp_path / "rhog_exp.p0.bin") and os.path.exists(tmp_path / "rho_exp_nr.p0.bin")
+    ref.close()

@@ -1,0 +1,278 @@
+"""Chains grafted to the plates and the BC/AC field pairs on the GPU, against
+the oracle (code/initialize.cpp:160-196 graft sites, calc_poly_density.cpp:
+131-160 graft propagators and density, 1s_update.cpp:57-329 pair updates,
+calc_h.cpp:9-64 H with the graft term).
+
+Cases: a small 2D box with the published chemistry (N = 20, Ng = 4, both
+signs of chi_BC), a 3D box with rod plates, and the two published 2D inputs
+as they are (525 x 875 and 625 x 675, N = 60, Ng = 9, sigma = 1.43,
+chi_BC = -0.018209, chi_AC = 1.13437).  FP64 to 1e-10; FP32 to 1e-6 in H;
+checkpointed FP64 equal to full storage.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import pyoracle
+import scft_amd
+from scft_amd import PREC_FP32, PREC_FP64, Workload
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+FIELDS = ["wpl", "wabp", "wabm", "wbcp", "wbcm", "wacp", "wacm"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def small_grafted(chi_bc=-0.018209, N=20, Ng=4):
+    p = scft_amd.read_input(os.path.join(GOLDEN, "bcp_dist-0.300.input"), 2)
+    p.Nda, p.Ndb = N // 2, N - N // 2
+    p.Ng = Ng
+    p.Nx[0], p.Nx[1] = 48, 80
+    p.L[0], p.L[1] = 24.0, 40.0
+    p.ic_flag[0] = 0
+    p.ic_flag[1] = 0
+    p.chi_bc = chi_bc
+    p.seed = 3
+    return p
+
+
+def grafted_3d():
+    w = Workload(grid=(16, 16, 32), box_rg=(2.5, 2.5, 5.0), N=20, np_type=scft_amd.NP_ROD, d=0.4,
+                 theta_deg=20.0, plate_side_rg=1.2, plate_thick_rg=0.25, plate_xi_rg=0.08)
+    p = w.params()
+    p.sigma = 0.4
+    p.Ng = 4
+    p.chi_bc = -0.018209 * 3
+    p.chi_ac = 1.13437 / 3
+    return p
+
+
+def physical_fields(p, seed, amp=0.05):
+    """Random fields in the saddle-point directions: w+ and the '+' fields
+    imaginary, the '-' fields real, swapped where chi < 0."""
+    rng = np.random.default_rng(seed)
+    out = []
+    chis = [None, p.chi_ab, p.chi_ab, p.chi_bc, p.chi_bc, p.chi_ac, p.chi_ac]
+    for k in range(7):
+        plus = k == 0 or k % 2 == 1
+        imag = plus if (chis[k] is None or chis[k] >= 0) else not plus
+        if k == 0:
+            imag = True
+        out.append(amp * rng.standard_normal(p.M) * (1j if imag else 1.0))
+    return out
+
+
+def compare_all(ctx, ref, p, tol, what, sys_=0):
+    for n in FIELDS:
+        a, b = ctx.array(n, sys=sys_), ref.array(n)
+        assert np.max(np.abs(a - b)) <= tol * max(np.max(np.abs(b)), 1e-3), (what, n, rel(a, b))
+    for n in ("rhoda", "rhodb", "rhog_exp"):
+        assert rel(ctx.array(n, sys=sys_), ref.array(n)) < tol, (what, n)
+    Qga = complex(ref.scalar("Qga_re"), ref.scalar("Qga_im"))
+    # Qga vanishes in zero fields (q_g = 1): relative to max(|Qga|, 1)
+    assert abs(ctx.get_Qga(sys=sys_) - Qga) <= tol * max(abs(Qga), 1.0), (what, ctx.get_Qga(sys=sys_), Qga)
+    assert abs(ctx.get_Q(sys=sys_) - ref.Q()) <= tol * abs(ref.Q()), what
+    H, Hr = ctx.calc_H()[sys_], ref.calc_H()
+    say(f" {what}: H {H.real:.12e} oracle {Hr.real:.12e} rel {abs(H - Hr) / abs(Hr):.2e}")
+    assert abs(H - Hr) <= tol * abs(Hr), what
+    return H
+
+
+@pytest.mark.parametrize("chi_bc", [-0.018209, 0.30])
+def test_small_grafted_fp64_matches_oracle(chi_bc):
+    p = small_grafted(chi_bc)
+    say("grafted 2D 48x80, chi_BC", chi_bc)
+    ctx = scft_amd.Context(p)
+    ref = pyoracle.Oracle(p)
+    info = ctx.info()
+    for k in ("phiD", "phiG", "ng_per_np", "nD"):
+        assert abs(getattr(info, k) - ref.scalar(k)) <= 1e-12 * abs(ref.scalar(k)), k
+    # graft sites at create
+    assert rel(ctx.array("expl_grafts"), ref.array("expl_grafts")) < 1e-12
+    f = physical_fields(p, 5)
+    for n, a in zip(FIELDS, f):
+        ctx.set_array(n, a)
+        ref.set_array(n, a)
+    ctx.calc_poly_density()
+    ref.calc_poly_density()
+    assert rel(ctx.array("smwg"), ref.array("smwg")) < 1e-12
+    for j in range(p.Ng):
+        assert rel(ctx.array(f"qg:{j}"), ref.array(f"qg:{j}")) < 1e-12, j
+    compare_all(ctx, ref, p, 1e-11, "density")
+    for it in range(3):
+        ctx.update_1s()
+        ref.update_1s()
+    compare_all(ctx, ref, p, 1e-10, "3 updates")
+    # graph-captured iterations continue the same sequence
+    ctx.iterate(5)
+    for it in range(5):
+        ref.update_1s()
+    compare_all(ctx, ref, p, 1e-10, "5 captured iterations")
+    ctx.close()
+    ref.close()
+
+
+def test_grafted_3d_rods_match_oracle():
+    p = grafted_3d()
+    say("grafted 3D 16x16x32 rods")
+    ctx = scft_amd.Context(p)
+    ref = pyoracle.Oracle(p)
+    assert rel(ctx.array("expl_grafts"), ref.array("expl_grafts")) < 1e-12
+    # start fields of init_fields (random, pairs included) from the same seed
+    for n in FIELDS:
+        assert np.max(np.abs(ctx.array(n) - ref.array(n))) <= 1e-14 * max(np.max(np.abs(ref.array(n))), 1.0), n
+    for n, a in zip(FIELDS, physical_fields(p, 8)):
+        ctx.set_array(n, a)
+        ref.set_array(n, a)
+    ctx.calc_poly_density()
+    ref.calc_poly_density()
+    compare_all(ctx, ref, p, 1e-11, "density")
+    for it in range(3):
+        ctx.update_1s()
+        ref.update_1s()
+    compare_all(ctx, ref, p, 1e-10, "3 updates")
+    # plates moved: graft sites follow
+    centres = np.array([0.35, 0.5, 0.5, 0.62, 0.5, 0.5])
+    us = np.array([0.3, 0.0, 1.0, 0.3, 0.0, 1.0])
+    ctx.set_plates(centres, us)
+    q = pyoracle.Oracle(_moved(p, centres, us))
+    assert rel(ctx.array("expl_grafts"), q.array("expl_grafts")) < 1e-12
+    assert abs(ctx.info().phiD - q.scalar("phiD")) < 1e-13
+    q.close()
+    ctx.close()
+    ref.close()
+
+
+def _moved(p, centres, us):
+    import copy
+    p2 = copy.copy(p)
+    for j in range(2):
+        m = np.linalg.norm(us[3 * j:3 * j + 3])
+        for d in range(3):
+            p2.exp_nr_c[j][d] = centres[3 * j + d]
+            p2.exp_nr_u[j][d] = us[3 * j + d] / m
+    return p2
+
+
+def test_small_grafted_fp32_and_checkpoint():
+    # FP32 propagators carry a relative error of about 1e-6 in Qd after N = 20
+    # steps (the same with and without grafts); H then moves by nD dQd/Qd and
+    # n ng_per_np dQga.  Bounds: dQd/Qd <= 3e-6, dQga <= 1e-6 Ng,
+    # dH <= 3e-6 nD + 1e-6 n ng_per_np Ng.  Checkpointed FP64 equals the
+    # oracle to 1e-11.
+    p = small_grafted()
+    f = physical_fields(p, 6)
+    say("grafted 2D FP32 / checkpoint")
+    ref = pyoracle.Oracle(p)
+    for n, a in zip(FIELDS, f):
+        ref.set_array(n, a)
+    ref.calc_poly_density()
+    Hd, Qd = ref.calc_H(), ref.Q()
+    Qga = complex(ref.scalar("Qga_re"), ref.scalar("Qga_im"))
+    for it in range(3):
+        ref.update_1s()
+    H3 = ref.calc_H()
+    for prec in (PREC_FP32, PREC_FP64 | scft_amd.OPT_CHECKPOINT, PREC_FP32 | scft_amd.OPT_CHECKPOINT):
+        ctx = scft_amd.Context(p, precision=prec)
+        info = ctx.info()
+        for n, a in zip(FIELDS, f):
+            ctx.set_array(n, a)
+        ctx.calc_poly_density()
+        h0 = ctx.calc_H()[0]
+        dQ = abs(ctx.get_Q() - Qd) / abs(Qd)
+        dQga = abs(ctx.get_Qga() - Qga)
+        for it in range(3):
+            ctx.update_1s()
+        h3 = ctx.calc_H()[0]
+        say(f" prec {prec:#x}: dQ/Q {dQ:.2e} dQga {dQga:.2e} dH {abs(h0 - Hd):.2e} after 3 {abs(h3 - H3):.2e}")
+        if prec & 0xff == PREC_FP64:
+            assert dQ < 1e-12 and dQga < 1e-12 * max(abs(Qga), 1.0)
+            assert abs(h0 - Hd) <= 1e-11 * abs(Hd) and abs(h3 - H3) <= 1e-10 * abs(H3)
+        else:
+            bound = 3e-6 * info.nD + 1e-6 * p.n_exp_nr * info.ng_per_np * p.Ng
+            assert dQ < 3e-6 and dQga < 1e-6 * p.Ng
+            assert abs(h0 - Hd) <= bound and abs(h3 - H3) <= bound
+        ctx.close()
+    ref.close()
+
+
+def test_pairs_reject_euler_and_noise():
+    p = small_grafted()
+    ctx = scft_amd.Context(p)
+    with pytest.raises(scft_amd.ScftError):
+        ctx.update_euler()
+    with pytest.raises(scft_amd.ScftError):
+        ctx.array("qg:99")
+    ctx.close()
+    p.do_CL = 1
+    with pytest.raises(scft_amd.ScftError):
+        scft_amd.Context(p)
+
+
+@pytest.mark.parametrize("name", ["bcp_dist-0.300.input", "bcp_angle-45.00.input"])
+def test_published_grafted_input_matches_oracle(name):
+    p = scft_amd.read_input(os.path.join(GOLDEN, name), 2)
+    p.seed = 1
+    say(name, "as published: grid", p.Nx[0], p.Nx[1], "N", p.N, "Ng", p.Ng, "sigma", p.sigma)
+    ctx = scft_amd.Context(p)
+    ref = pyoracle.Oracle(p)
+    info = ctx.info()
+    for k in ("V_1_exp_np", "np_frac", "phiD", "phiG", "ng_per_np", "nD"):
+        assert abs(getattr(info, k) - ref.scalar(k)) <= 1e-12 * abs(ref.scalar(k)), k
+    assert rel(ctx.array("expl_grafts"), ref.array("expl_grafts")) < 1e-12
+    for n in FIELDS:
+        assert np.max(np.abs(ctx.array(n) - ref.array(n))) <= 1e-14 * max(np.max(np.abs(ref.array(n))), 1.0), n
+    ctx.calc_poly_density()
+    ref.calc_poly_density()
+    compare_all(ctx, ref, p, 1e-11, "density")
+    for it in range(2):
+        ctx.update_1s()
+        ref.update_1s()
+    compare_all(ctx, ref, p, 1e-10, "2 updates")
+    ctx.iterate(2)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.iterate(20)
+    ctx.synchronize()
+    say(f" {name}: {20 / (time.perf_counter() - t0):.1f} iter/s (FP64, grafted)")
+    ctx.close()
+    ref.close()
+
+
+def test_published_log_bookkeeping():
+    # the set-up scalars every published LOG prints (code/initialize.cpp:
+    # 371-385), for eight of the 52 published runs, from the library
+    import json
+    import tempfile
+    runs = json.load(open(os.path.join(GOLDEN, "log_known_answers.json")))["runs"]
+    os.environ["SCFT_NO_TUNE"] = "1"
+    try:
+        with tempfile.TemporaryDirectory() as td:
+            for r in runs[::7]:
+                path = os.path.join(td, "bcp.input")
+                open(path, "w").write(r["input"])
+                p = scft_amd.read_input(path, 2)
+                say(" bookkeeping", r["sweep"], r["run"])
+                ctx = scft_amd.Context(p)
+                f, log = ctx.info(), r["log"]
+                vseg = f.nD * f.N + f.nP * f.ng_per_np * p.Ng
+                for got, key in ((f.V_1_exp_np, "V_1_exp_np"), (f.nD, "nD"), (f.np_frac, "np_frac"),
+                                 (f.phiG, "phiG"), (f.ng_per_np, "ng_per_np"), (f.V, "V"), (vseg, "Vseg"),
+                                 (f.nD * f.N, "nDN")):
+                    # printed with %lf: six decimals
+                    assert abs(got - log[key]) <= 5.01e-7 + 1e-12 * abs(log[key]), (r["run"], key, got, log[key])
+                ctx.close()
+    finally:
+        del os.environ["SCFT_NO_TUNE"]
