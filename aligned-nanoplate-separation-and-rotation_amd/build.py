@@ -20,7 +20,7 @@ DRIVER = os.path.join(HERE, "scft_run")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["scft_kernels.hip", "conv_kernels.hip", "scft_api.hip", "input.cpp"]
+SOURCES = ["scft_kernels.hip", "conv_kernels.hip", "odd_kernels.hip", "scft_api.hip", "input.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

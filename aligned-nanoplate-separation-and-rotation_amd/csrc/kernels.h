@@ -20,6 +20,11 @@ hipError_t launch_fast_plane(int prec, int mode, int variant, const PassDesc* d,
                              int step, hipStream_t st);
 hipError_t launch_fast_z(int prec, int mode, int variant, const PassDesc* d, int njobs, int gridx, int step,
                          hipStream_t st);
+// line passes for the odd published lengths (odd_kernels.hip): variant ids
+// and their lines per block for a length and axis, then launches by id
+int odd_line_variants(int prec, int n, int axis, int* ids, int* cpb, int max_ids);
+hipError_t launch_odd_line(int prec, int mode, int variant, const PassDesc* d, int njobs, int gridx, int step,
+                           hipStream_t st);
 hipError_t launch_copy16(const void* src, void* dst, long n16, hipStream_t st);
 hipError_t launch_assemble(const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_exp_prop(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);

@@ -258,6 +258,11 @@ int build_program(scft_ctx* c, int mode, const std::vector<JobSpec>& jobs, Progr
       } else if (kind == ST_Z && scft::fast_z_variants(jobs[j].prec, s.nz, ids, tls, 16) > 0) {
         fast = 2;
         n0 = s.nz;
+      } else if ((kind == ST_X || kind == ST_Y) &&
+                 scft::odd_line_variants(jobs[j].prec, kind == ST_X ? s.nx : s.ny, kind - ST_X, ids, tls, 16) > 0) {
+        fast = 3;
+        n0 = kind == ST_X ? s.nx : s.ny;
+        n1 = kind - ST_X;
       }
       groups[{(int)k, kind, jobs[j].prec, fast, n0, n1}].push_back(
           {(int)j, (int)k, kind, k + 1 == seq[j].size()});
@@ -277,11 +282,12 @@ int build_program(scft_ctx* c, int mode, const std::vector<JobSpec>& jobs, Progr
     L.n1 = std::get<5>(kv.first);
     if (L.fast) {
       int ids[16], tls[16];
-      const int nv = L.fast == 1 ? scft::fast_plane_variants(prec, L.n0, L.n1, ids, 16)
-                                 : scft::fast_z_variants(prec, L.n0, ids, tls, 16);
+      const int nv = L.fast == 1   ? scft::fast_plane_variants(prec, L.n0, L.n1, ids, 16)
+                     : L.fast == 2 ? scft::fast_z_variants(prec, L.n0, ids, tls, 16)
+                                   : scft::odd_line_variants(prec, L.n0, L.n1, ids, tls, 16);
       for (int v = 0; v < nv; v++) {
         L.variants.push_back(ids[v]);
-        L.tiles.push_back(L.fast == 2 ? tls[v] : 0);
+        L.tiles.push_back(L.fast >= 2 ? tls[v] : 0);
       }
     }
     L.mode = mode;
@@ -337,7 +343,7 @@ int build_program(scft_ctx* c, int mode, const std::vector<JobSpec>& jobs, Progr
         d.tw0 = tw_ptr(c, n, prec);
         d.g0 = gsel(axis);
         const int nlines = s.M / n;
-        if (L.fast == 2) {
+        if (L.fast >= 2) {
           L.max_lines = std::max(L.max_lines, nlines);
           all.push_back(d);
           continue;
@@ -391,6 +397,11 @@ hipError_t launch_one(scft_ctx* c, const Launch& L, int step, hipStream_t st = n
     return scft::launch_fast_z(L.prec, L.mode, L.variants[L.choice], L.d, L.njobs, (L.max_lines + tl - 1) / tl,
                                step, st);
   }
+  if (L.fast == 3) {
+    const int cpb = L.tiles[L.choice];
+    return scft::launch_odd_line(L.prec, L.mode, L.variants[L.choice], L.d, L.njobs,
+                                 (L.max_lines + cpb - 1) / cpb, step, st);
+  }
   return scft::launch_pass(L.prec, L.mode, L.plane, L.d, L.njobs, L.gridx, L.tb, L.lds, step, st);
 }
 
@@ -402,8 +413,15 @@ int autotune(scft_ctx* c, Program& p, int step) {
   hipEvent_t e0, e1;
   HIPCK(hipEventCreate(&e0));
   HIPCK(hipEventCreate(&e1));
+  // SCFT_TUNE_PICK=k: take variant k (or the last) of every launch instead
+  // of timing them, so that tests can run each variant
+  const char* pick = getenv("SCFT_TUNE_PICK");
   for (Launch& L : p.launches) {
     if (L.variants.size() < 2) continue;
+    if (pick) {
+      L.choice = std::min(std::max(atoi(pick), 0), (int)L.variants.size() - 1);
+      continue;
+    }
     float best = 1e30f;
     int best_i = 0;
     for (size_t v = 0; v < L.variants.size(); v++) {
@@ -1851,6 +1869,9 @@ int scft_describe(scft_ctx* c, char* buf, int len) {
       else if (L.fast == 2)
         snprintf(tmp, sizeof tmp, " z_conv[%d tile%d prec%d jobs%d variant %d of %zu]", L.n0, L.tiles[L.choice],
                  L.prec, L.njobs, L.variants[L.choice], L.variants.size());
+      else if (L.fast == 3)
+        snprintf(tmp, sizeof tmp, " odd_line[%d axis%d lines%d mode%d prec%d jobs%d variant %d of %zu]", L.n0,
+                 L.n1, L.tiles[L.choice], L.mode, L.prec, L.njobs, L.variants[L.choice], L.variants.size());
       else
         snprintf(tmp, sizeof tmp, " %s[mode%d prec%d jobs%d tb%d lds%zu]", L.plane ? "plane" : "line", L.mode,
                  L.prec, L.njobs, L.tb, L.lds);

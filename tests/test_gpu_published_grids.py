@@ -1,5 +1,5 @@
 """The published 2D grids (525 x 875 separation runs, 625 x 675 angle runs;
-radices 3, 5, 7) through the generic passes, against the oracle.
+radices 3, 5, 7) through the odd-length line passes, against the oracle.
 
 The published inputs (tests/golden/bcp_dist-0.300.input,
 bcp_angle-45.00.input) have grafted chains, which this path does not build;
@@ -79,4 +79,37 @@ def test_published_grid_iteration_matches_oracle(name):
     ctx.synchronize()
     say(f" {name}: {20 / (time.perf_counter() - t0):.1f} iter/s (FP64)")
     ctx.close()
+    ref.close()
+
+
+@pytest.mark.parametrize("name", ["bcp_dist-0.300.input", "bcp_angle-45.00.input"])
+@pytest.mark.parametrize("pick", [0, 1, 2])
+def test_every_odd_line_variant(name, pick):
+    # SCFT_TUNE_PICK fixes the variant of every multi-variant launch: each
+    # lines-per-block / lanes-per-line shape of the x and y passes, in FP64
+    # and FP32, through the FFT seam and one density evaluation
+    p = graft_free(name)
+    say(name, "variant pick", pick)
+    ref = pyoracle.Oracle(p)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(p.M) + 1j * rng.standard_normal(p.M)
+    ref.calc_poly_density()
+    os.environ["SCFT_TUNE_PICK"] = str(pick)
+    try:
+        for prec, tol in ((scft_amd.PREC_FP64, 1e-12), (scft_amd.PREC_FP32, 2e-5)):
+            ctx = scft_amd.Context(p, precision=prec)
+            plan = ctx.describe()
+            assert "odd_line[" in plan, plan
+            if prec == scft_amd.PREC_FP64:
+                assert rel(ctx.fft(x, -1), ref.fft(x, -1)) < 1e-13
+                assert rel(ctx.fft(x, +1), ref.fft(x, +1)) < 1e-13
+            ctx.calc_poly_density()
+            dq = abs(ctx.get_Q() - ref.Q()) / abs(ref.Q())
+            rA, rB = ctx.get_density()
+            da, db = rel(rA, ref.array("rhoda")), rel(rB, ref.array("rhodb"))
+            say(f" prec {prec}: dQ/Q {dq:.2e} drhoA {da:.2e} drhoB {db:.2e}")
+            assert dq < tol and da < tol and db < tol
+            ctx.close()
+    finally:
+        del os.environ["SCFT_TUNE_PICK"]
     ref.close()
