@@ -258,6 +258,177 @@ hipError_t launch_odd_t(int mode, const PassDesc* d, int njobs, int gridx, int s
   return launch_odd_m<R, MODE_INV, N1, N2, AXIS, CPB, LW>(d, njobs, gridx, step, st);
 }
 
+// ---------------------------------------------------------------------------
+// Stockham variant: the line transform as a sequence of radix-R passes in
+// LDS with compile-time radices, one butterfly per lane per pass (lanes per
+// line = max n / R), so each lane does a few small DFTs instead of two large
+// ones.  Pass p with radix R and NS = product of the earlier radices maps
+// butterfly j to in x[j + r n/R], twiddle exp(sign 2 pi i r (j mod NS) /
+// (NS R)), out y[(j / NS) NS R + (j mod NS) + r NS] (autosort Stockham, as
+// lds_fft.hpp does with run-time radices).
+// ---------------------------------------------------------------------------
+template <int N>
+struct RadixPlan {
+  int np;
+  int r[16];
+  constexpr RadixPlan() : np(0), r() {
+    int m = N;
+    while (m > 1) {
+      const int p = m % 5 == 0 ? 5 : m % 3 == 0 ? 3 : m % 7 == 0 ? 7 : m % 4 == 0 ? 4 : 2;
+      r[np++] = p;
+      m /= p;
+    }
+  }
+  constexpr int max_bfly() const {
+    int b = 0;
+    for (int i = 0; i < np; i++) b = N / r[i] > b ? N / r[i] : b;
+    return b;
+  }
+};
+template <int N>
+struct PlanTab {
+  static constexpr RadixPlan<N> v = RadixPlan<N>();
+};
+
+template <int N, int AXIS, int CPB>
+struct StockCfg {
+  static constexpr int TPL = PlanTab<N>::v.max_bfly();  // lanes per line
+  static constexpr int T = ((CPB * TPL + 63) / 64) * 64;
+};
+
+// passes P.. of one transform; g (if non-null) multiplies the last pass's
+// outputs at their natural frequency index
+template <typename R, int SIGN, int N, int P, int NS>
+__device__ __forceinline__ void stock_passes(cpx<R>* __restrict__ L, const cpx<R>* __restrict__ tw,
+                                             const R* __restrict__ g, int lane, bool on) {
+  constexpr int NP = PlanTab<N>::v.np;
+  if constexpr (P < NP) {
+    constexpr int RAD = PlanTab<N>::v.r[P];
+    constexpr int NB = N / RAD;
+    constexpr int TWS = N / (NS * RAD);
+    const bool act = on && lane < NB;
+    cpx<R> a[RAD];
+    const int j = lane, k = j % NS;
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < RAD; r++) a[r] = L[j + r * NB];
+    }
+    __syncthreads();
+    if (act) {
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < RAD; r++) {
+          cpx<R> t = tw[r * k * TWS];
+          if (SIGN > 0) t.y = -t.y;
+          a[r] = cmul(a[r], t);
+        }
+      }
+      dft<RAD, SIGN>(a);
+      const int base = (j / NS) * NS * RAD + k;
+#pragma unroll
+      for (int r = 0; r < RAD; r++) {
+        const int i = base + r * NS;
+        cpx<R> o = a[r];
+        if (P + 1 == NP && g) o = cscale(o, g[i]);
+        L[i] = o;
+      }
+    }
+    __syncthreads();
+    stock_passes<R, SIGN, N, P + 1, NS * RAD>(L, tw, g, lane, on);
+  }
+}
+
+template <typename R, int MODE, int N, int AXIS, int CPB>
+__global__ __launch_bounds__((StockCfg<N, AXIS, CPB>::T)) void k_stock_line(const PassDesc* __restrict__ descs,
+                                                                           int step) {
+  using C = StockCfg<N, AXIS, CPB>;
+  constexpr int TPL = C::TPL;
+  __shared__ cpx<R> lds[CPB * N];
+  const PassDesc& d = descs[blockIdx.y];
+  if (d.steps_active && step_contour(step) >= d.steps_active) return;
+  const long M = (long)d.nx * d.ny * d.nz;
+  const int nlines = (int)(M / N);
+  if ((int)blockIdx.x * CPB >= nlines) return;
+  const int t = threadIdx.x;
+  int c, lane;
+  if constexpr (AXIS == 0) {
+    c = t / TPL;
+    lane = t - c * TPL;
+  } else {
+    c = t % CPB;
+    lane = t / CPB;
+  }
+  const int line = blockIdx.x * CPB + c;
+  const bool on = c < CPB && lane < TPL && line < nlines;
+  long base, S;
+  if constexpr (AXIS == 0) {
+    base = (long)line * N;
+    S = 1;
+  } else {
+    base = (long)(line % d.nx) + (long)d.nx * d.ny * (line / d.nx);
+    S = d.nx;
+  }
+  const cpx<R>* in = reinterpret_cast<const cpx<R>*>(d.in) + (long)(step_slice(step) + d.in_soff) * d.in_step;
+  cpx<R>* out = reinterpret_cast<cpx<R>*>(d.out) + (long)(step_slice(step) + d.out_soff) * d.out_step;
+  const cpx<R>* tw = reinterpret_cast<const cpx<R>*>(d.tw0);
+  const R* g = MODE == MODE_CONV ? reinterpret_cast<const R*>(d.g0) : nullptr;
+  cpx<R>* L = lds + (c < CPB ? c : 0) * N;
+  if (on) {
+    for (int i = lane; i < N; i += TPL) L[i] = in[base + (long)i * S];
+  }
+  __syncthreads();
+  if constexpr (MODE == MODE_CONV) {
+    stock_passes<R, -1, N, 0, 1>(L, tw, g, lane, on);
+    stock_passes<R, +1, N, 0, 1>(L, tw, nullptr, lane, on);
+  } else if constexpr (MODE == MODE_FWD) {
+    stock_passes<R, -1, N, 0, 1>(L, tw, nullptr, lane, on);
+  } else {
+    stock_passes<R, +1, N, 0, 1>(L, tw, nullptr, lane, on);
+  }
+  const cpx<R>* post = nullptr;
+  R scale = R(1);
+  if (d.last) {
+    const void* pp = (step_contour(step) < d.post_switch) ? d.post_a : d.post_b;
+    if (pp) post = reinterpret_cast<const cpx<R>*>(pp);
+    scale = R(d.scale);
+  }
+  if (on) {
+    for (int i = lane; i < N; i += TPL) {
+      const long gi = base + (long)i * S;
+      cpx<R> o = L[i];
+      if (post) o = cmul(o, post[gi]);
+      if (scale != R(1)) o = cscale(o, scale);
+      out[gi] = o;
+    }
+  }
+}
+
+template <typename R, int MODE, int N, int AXIS, int CPB>
+hipError_t launch_stock_m(const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
+  using C = StockCfg<N, AXIS, CPB>;
+  hipLaunchKernelGGL((k_stock_line<R, MODE, N, AXIS, CPB>), dim3(gridx, njobs), dim3(C::T), 0, st, d, step);
+  return hipGetLastError();
+}
+
+template <int NN, int AX, int CP>
+hipError_t launch_stock_v(int prec, int mode, const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
+  if constexpr ((size_t)CP * NN * 16 <= 64 * 1024) {
+    if (prec == 0) {
+      if (mode == MODE_CONV) return launch_stock_m<double, MODE_CONV, NN, AX, CP>(d, njobs, gridx, step, st);
+      if (mode == MODE_FWD) return launch_stock_m<double, MODE_FWD, NN, AX, CP>(d, njobs, gridx, step, st);
+      return launch_stock_m<double, MODE_INV, NN, AX, CP>(d, njobs, gridx, step, st);
+    }
+  }
+  if constexpr ((size_t)CP * NN * 8 <= 64 * 1024) {
+    if (prec == 1) {
+      if (mode == MODE_CONV) return launch_stock_m<float, MODE_CONV, NN, AX, CP>(d, njobs, gridx, step, st);
+      if (mode == MODE_FWD) return launch_stock_m<float, MODE_FWD, NN, AX, CP>(d, njobs, gridx, step, st);
+      return launch_stock_m<float, MODE_INV, NN, AX, CP>(d, njobs, gridx, step, st);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
 // the precisions whose static LDS (CPB lines of n values) fits 64 KiB
 template <int NN, int N1, int N2, int AX, int CP, int LW>
 hipError_t launch_odd_v(int prec, int mode, const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
@@ -283,6 +454,12 @@ hipError_t launch_odd_v(int prec, int mode, const PassDesc* d, int njobs, int gr
   X(675, 25, 27, 1, 2, 0) X(675, 25, 27, 1, 4, 0) X(675, 25, 27, 1, 8, 0)                            \
   X(875, 35, 25, 1, 2, 0) X(875, 35, 25, 1, 4, 0) X(875, 35, 25, 1, 8, 0)
 
+// S(n, axis, lines per block): Stockham variants (block size <= 1024)
+#define SCFT_STOCK_VARIANTS(S)                                                                       \
+  S(625, 0, 1) S(625, 0, 2) S(525, 0, 1) S(525, 0, 2) S(675, 0, 1) S(675, 0, 2) S(875, 0, 1)         \
+  S(875, 0, 2) S(625, 1, 2) S(625, 1, 4) S(625, 1, 8) S(525, 1, 2) S(525, 1, 4) S(675, 1, 2)         \
+  S(675, 1, 4) S(875, 1, 2) S(875, 1, 4)
+
 int odd_line_variants(int prec, int n, int axis, int* ids, int* cpb, int max_ids) {
   int k = 0, id = 0;
   const size_t esz = prec == 0 ? 16 : 8;
@@ -295,6 +472,15 @@ int odd_line_variants(int prec, int n, int axis, int* ids, int* cpb, int max_ids
   id++;
   SCFT_ODD_VARIANTS(X)
 #undef X
+#define S(nn, ax, cp)                                                             \
+  if (n == nn && axis == ax && k < max_ids && (size_t)cp * nn * esz <= 64 * 1024 && \
+      StockCfg<nn, ax, cp>::T <= 1024) {                                          \
+    cpb[k] = cp;                                                                  \
+    ids[k++] = id;                                                                \
+  }                                                                               \
+  id++;
+  SCFT_STOCK_VARIANTS(S)
+#undef S
   return k;
 }
 
@@ -306,6 +492,15 @@ hipError_t launch_odd_line(int prec, int mode, int variant, const PassDesc* d, i
   id++;
   SCFT_ODD_VARIANTS(X)
 #undef X
+#define S(nn, ax, cp)                                                                             \
+  if (variant == id) {                                                                            \
+    if constexpr (StockCfg<nn, ax, cp>::T <= 1024)                                                \
+      return launch_stock_v<nn, ax, cp>(prec, mode, d, njobs, gridx, step, st);                   \
+    return hipErrorInvalidValue;                                                                  \
+  }                                                                                               \
+  id++;
+  SCFT_STOCK_VARIANTS(S)
+#undef S
   return hipErrorInvalidValue;
 }
 

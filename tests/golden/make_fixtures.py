@@ -17,8 +17,10 @@ Outputs
                            e^{-i} and backward e^{+i}, unnormalised
   log_templates.json       the line sequence of two published LOGs with every
                            number replaced by '#' (logfmt.log_template)
+  log_last_rows.json       the last three main-loop lines of every published
+                           LOG: iter, H, -log(Qd), -log(Qg), err
 
-    python tests/golden/make_fixtures.py --templates   # log_templates.json only
+    python tests/golden/make_fixtures.py --templates   # the two LOG-line fixtures only
 """
 import glob
 import json
@@ -60,6 +62,23 @@ def write_log_templates():
         runs[run] = log_template(open(os.path.join(REF, sweep, run, "LOG")).read())
     with open(os.path.join(HERE, "log_templates.json"), "w") as f:
         json.dump(runs, f, indent=1)
+
+
+def last_rows(path, k=3):
+    """The last k main-loop lines of a LOG: iter, H, -log(Qd), -log(Qg), err."""
+    pat = re.compile(r"Iter: (\d+), H=(\S+), -log\(Qd\)=(\S+), -log\(Qg\)=(\S+), err=(\S+)")
+    rows = [[float(x) for x in m.groups()] for m in map(pat.match, open(path)) if m]
+    return rows[-k:]
+
+
+def write_last_rows():
+    out = {}
+    for sweep in ("particle-separation", "angle-rotation"):
+        for d in sorted(glob.glob(os.path.join(REF, sweep, "*-*"))):
+            if os.path.isdir(d):
+                out[os.path.basename(d)] = last_rows(os.path.join(d, "LOG"))
+    with open(os.path.join(HERE, "log_last_rows.json"), "w") as f:
+        json.dump(out, f, indent=1)
 
 
 def parse_hamiltonians(path):
@@ -115,6 +134,7 @@ def main():
         vec[f"A{key}"] = np.fft.fftn(a)
     np.savez_compressed(os.path.join(HERE, "fft_vectors.npz"), **vec)
     write_log_templates()
+    write_last_rows()
     print("wrote", len(runs), "runs and", len(vec), "fft arrays")
 
 
@@ -122,5 +142,6 @@ if __name__ == "__main__":
     import sys
     if "--templates" in sys.argv:
         write_log_templates()
+        write_last_rows()
     else:
         main()

@@ -319,3 +319,38 @@ def test_driver_grafted_run_matches_oracle(tmp_path):
         assert np.all(scft_amd.read_bin(str(tmp_path / f"{n}.p0.bin"))[4] == 0)
     assert os.path.exists(tmp_path / "rhog_exp.p0.bin") and os.path.exists(tmp_path / "rho_exp_nr.p0.bin")
     ref.close()
+
+
+@pytest.mark.gpu
+def test_driver_reproduces_published_run(tmp_path):
+    # the published angle-00.00 input as it is, started from its own
+    # init_fields flags (the published run resumed from .res fields that the
+    # snapshot lacks): the reference's stop rule ends it, and its last three
+    # printed rows match the published LOG's last three rows to the print
+    # precision in H and -log(Qd), and to 5e-5 in -log(Qg)
+    need_driver()
+    import json
+    sys.path.insert(0, GOLDEN)
+    try:
+        from published_input import published
+    finally:
+        sys.path.remove(GOLDEN)
+    text, H_pub = published("angle-00.00")
+    (tmp_path / "bcp.input").write_text(text)
+    want = json.load(open(os.path.join(GOLDEN, "log_last_rows.json")))["angle-00.00"]
+    print("scft_run published angle-00.00", flush=True)
+    r = subprocess.run([DRIVER, "--dim", "2", "--no-bin"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Tolerance reached" in r.stdout
+    rows = [l for l in r.stdout.splitlines() if l.startswith("Iter: ")][-3:]
+    got = []
+    for l in rows:
+        f = dict(kv.split("=") for kv in l.split(", ")[1:])
+        got.append([float(f["H"]), float(f["-log(Qd)"]), float(f["-log(Qg)"]), float(f["err"])])
+    print(" last rows", rows, flush=True)
+    for g, w in zip(got, want):
+        assert abs(g[0] - w[1]) <= 2.01e-6, (g, w)
+        assert abs(g[1] - w[2]) <= 2.01e-6, (g, w)
+        assert abs(g[2] - w[3]) <= 5e-5, (g, w)
+    assert abs(got[-1][0] - H_pub) <= 2.01e-6

@@ -83,11 +83,12 @@ def test_published_grid_iteration_matches_oracle(name):
 
 
 @pytest.mark.parametrize("name", ["bcp_dist-0.300.input", "bcp_angle-45.00.input"])
-@pytest.mark.parametrize("pick", [0, 1, 2])
+@pytest.mark.parametrize("pick", [0, 1, 2, 3, 4, 5])
 def test_every_odd_line_variant(name, pick):
-    # SCFT_TUNE_PICK fixes the variant of every multi-variant launch: each
-    # lines-per-block / lanes-per-line shape of the x and y passes, in FP64
-    # and FP32, through the FFT seam and one density evaluation
+    # SCFT_TUNE_PICK fixes the variant of every multi-variant launch (the
+    # last one for picks beyond a launch's count): each two-step and
+    # Stockham shape of the x and y passes, in FP64 and FP32, through the FFT
+    # seam and one density evaluation
     p = graft_free(name)
     say(name, "variant pick", pick)
     ref = pyoracle.Oracle(p)
