@@ -108,6 +108,16 @@ __device__ __forceinline__ void reg_dft(cpx<R>* a) {
   }
 }
 
+// Column-group index of a block of a strided (y) pass.  Workgroups are
+// dispatched to the 8 XCDs in turn (block b on XCD b mod 8), and each XCD
+// has its own L2.  Neighbouring column groups share 128-byte lines, so the
+// blocks of one XCD get one contiguous run of groups: XCD x takes groups
+// [x q + min(x, r), ...) with nb = 8 q + r.  A bijection of [0, nb).
+__device__ __forceinline__ int xcd_group(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
 template <int N1, int N2, int AXIS, int CPB, int LW>
 struct OddCfg {
   static constexpr int n = N1 * N2;
@@ -134,7 +144,8 @@ __global__ __launch_bounds__((OddCfg<N1, N2, AXIS, CPB, LW>::T)) void k_odd_line
   if (d.steps_active && step_contour(step) >= d.steps_active) return;
   const long M = (long)d.nx * d.ny * d.nz;
   const int nlines = (int)(M / n);
-  if ((int)blockIdx.x * CPB >= nlines) return;
+  const int bx = AXIS == 0 ? (int)blockIdx.x : xcd_group(blockIdx.x, gridDim.x);
+  if (bx * CPB >= nlines) return;
   const int t = threadIdx.x;
   int c, j;
   if constexpr (AXIS == 0) {
@@ -144,7 +155,7 @@ __global__ __launch_bounds__((OddCfg<N1, N2, AXIS, CPB, LW>::T)) void k_odd_line
     c = t % CPB;
     j = t / CPB;
   }
-  const int line = blockIdx.x * CPB + c;
+  const int line = bx * CPB + c;
   const bool live = line < nlines;
   long base, S;
   if constexpr (AXIS == 0) {
@@ -348,7 +359,8 @@ __global__ __launch_bounds__((StockCfg<N, AXIS, CPB>::T)) void k_stock_line(cons
   if (d.steps_active && step_contour(step) >= d.steps_active) return;
   const long M = (long)d.nx * d.ny * d.nz;
   const int nlines = (int)(M / N);
-  if ((int)blockIdx.x * CPB >= nlines) return;
+  const int bx = AXIS == 0 ? (int)blockIdx.x : xcd_group(blockIdx.x, gridDim.x);
+  if (bx * CPB >= nlines) return;
   const int t = threadIdx.x;
   int c, lane;
   if constexpr (AXIS == 0) {
@@ -358,7 +370,7 @@ __global__ __launch_bounds__((StockCfg<N, AXIS, CPB>::T)) void k_stock_line(cons
     c = t % CPB;
     lane = t / CPB;
   }
-  const int line = blockIdx.x * CPB + c;
+  const int line = bx * CPB + c;
   const bool on = c < CPB && lane < TPL && line < nlines;
   long base, S;
   if constexpr (AXIS == 0) {

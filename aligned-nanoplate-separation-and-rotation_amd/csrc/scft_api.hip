@@ -154,6 +154,9 @@ struct scft_ctx {
   // fill the other's gaps and tails
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // third branch: the graft chains, concurrent with the diblock chains
+  hipStream_t st3 = nullptr;
+  hipEvent_t ev_gjoin = nullptr;
   bool split_chains = false;
   std::vector<SysHost> sys;
   SysDev* d_sys = nullptr;
@@ -1086,17 +1089,17 @@ int build_programs(scft_ctx* c) {
 // Chains grafted to the plates, code/calc_poly_density.cpp:131-133, 155-160:
 // free-end propagator, complementary propagator from the graft sites,
 // graft density and the log-average Qga_exp
-int stage_graft(scft_ctx* c) {
+int stage_graft(scft_ctx* c, hipStream_t st) {
   if (!c->any_graft) return SCFT_OK;
   const int ns = (int)c->sys.size();
   int rc;
   for (int step = 1; step < c->maxNg; step++)
-    if ((rc = run_program(c, c->prog_graft_q, step))) return rc;
-  HIPCK(scft::launch_graft_init(c->prec, c->d_sys, ns, c->maxM, c->st));
+    if ((rc = run_program(c, c->prog_graft_q, step, st))) return rc;
+  HIPCK(scft::launch_graft_init(c->prec, c->d_sys, ns, c->maxM, st));
   for (int step = 1; step < c->maxNg; step++)
-    if ((rc = run_program(c, c->prog_graft_qd, step))) return rc;
-  HIPCK(scft::launch_graft_density(c->prec, c->d_sys, ns, c->maxM, c->st));
-  HIPCK(scft::launch_reduce(scft::RED_QGA, c->prec, c->d_sys, ns, c->nblk, c->st));
+    if ((rc = run_program(c, c->prog_graft_qd, step, st))) return rc;
+  HIPCK(scft::launch_graft_density(c->prec, c->d_sys, ns, c->maxM, st));
+  HIPCK(scft::launch_reduce(scft::RED_QGA, c->prec, c->d_sys, ns, c->nblk, st));
   return SCFT_OK;
 }
 
@@ -1163,17 +1166,29 @@ int stage_poly_density(scft_ctx* c) {
   HIPCK(scft::launch_exp_prop(c->prec, c->d_sys, ns, c->maxM, c->st));
   if (c->ckpt) {
     if ((rc = stage_ck_density(c))) return rc;
-    return stage_graft(c);
+    return stage_graft(c, c->st);
   }
   if (c->split_chains) {
+    // graph branches: q on st, q-dagger on st2, the graft chains on st3
+    // (they need only the smeared graft field; their reduction has its own
+    // slot of the partials)
     HIPCK(hipEventRecord(c->ev_fork, c->st));
     HIPCK(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+    if (c->any_graft) {
+      HIPCK(hipStreamWaitEvent(c->st3, c->ev_fork, 0));
+      if ((rc = stage_graft(c, c->st3))) return rc;
+      HIPCK(hipEventRecord(c->ev_gjoin, c->st3));
+    }
     for (int step = 1; step < c->maxN; step++) {
       if ((rc = run_program(c, c->prog_prop_q, step, c->st))) return rc;
       if ((rc = run_program(c, c->prog_prop_qd, step, c->st2))) return rc;
     }
     HIPCK(hipEventRecord(c->ev_join, c->st2));
     HIPCK(hipStreamWaitEvent(c->st, c->ev_join, 0));
+    if (c->any_graft) HIPCK(hipStreamWaitEvent(c->st, c->ev_gjoin, 0));
+    HIPCK(scft::launch_reduce(scft::RED_Q, c->prec, c->d_sys, ns, c->nblk, c->st));
+    HIPCK(scft::launch_density(c->prec, c->d_sys, ns, c->maxM, c->st));
+    return SCFT_OK;
   } else {
     for (int step = 1; step < c->maxN; step++) {
       rc = run_program(c, c->prog_prop, step);
@@ -1182,7 +1197,7 @@ int stage_poly_density(scft_ctx* c) {
   }
   HIPCK(scft::launch_reduce(scft::RED_Q, c->prec, c->d_sys, ns, c->nblk, c->st));
   HIPCK(scft::launch_density(c->prec, c->d_sys, ns, c->maxM, c->st));
-  return stage_graft(c);
+  return stage_graft(c, c->st);
 }
 
 int stage_update(scft_ctx* c, int scheme) {
@@ -1359,8 +1374,10 @@ int scft_create(const scft_params* params, int n_systems, int device, int precis
   }
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->st3, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_gjoin, hipEventDisableTiming) != hipSuccess) {
     scft_destroy(c);
     return fail(SCFT_EDEVICE, "hipStreamCreate failed");
   }
@@ -1461,18 +1478,21 @@ int scft_destroy(scft_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
   if (c->st2) (void)hipStreamSynchronize(c->st2);
+  if (c->st3) (void)hipStreamSynchronize(c->st3);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   for (auto& s : c->sys)
     for (void* p : s.allocs) (void)hipFree(p);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->st) (void)hipStreamDestroy(c->st);
   if (c->st2) (void)hipStreamDestroy(c->st2);
+  if (c->st3) (void)hipStreamDestroy(c->st3);
   for (hipEvent_t e : c->ck_fork)
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->ck_join)
     if (e) (void)hipEventDestroy(e);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_gjoin) (void)hipEventDestroy(c->ev_gjoin);
   delete c;
   return SCFT_OK;
 }
