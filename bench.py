@@ -6,7 +6,9 @@ Headline (`value`): the FP32-propagator path on a batch of 16 production
 systems (separations d = 0.14 ... 0.44) held in one context, in
 system-iterations/s; each step advances all 16 systems by one iteration.
 Beside it: the single production system with the FP32 propagator and with
-the plain FP64 path, their ratio, and |dH/H| between the two.
+the plain FP64 path, their ratio, and |dH/H| between the two; and
+(`published_2d`) the rates of two published 2D runs as published (525 x 875
+and 625 x 675 grids, N = 60, grafted chains).
 
 A step is one SCFT iteration of the production system (64 x 64 x 128 grid,
 box 4 x 4 x 8 Rg, AB diblock N = 100, f_A = 0.5, chi N = 20, C = 50,
@@ -119,6 +121,36 @@ def run_batch(ws, prec, warmup, steps):
     return ctx, dt, ps
 
 
+def published_2d(iters=200):
+    """Rates of two published 2D runs as published (525 x 875 and 625 x 675,
+    N = 60, grafted chains, the input's own start fields): FP64 and FP32
+    iterations/s of one system, with the B_iter fraction of HBM (B_iter of
+    the diblock path; the Ng-step graft chains are not counted)."""
+    import tempfile
+    sys.path.insert(0, os.path.join(REPO, "tests", "golden"))
+    from published_input import published
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for run in ("angle-05.00", "dist-0.300"):
+            path = os.path.join(td, "bcp.input")
+            open(path, "w").write(published(run)[0])
+            p = scft_amd.read_input(path, 2)
+            for name, prec in (("fp64", scft_amd.PREC_FP64), ("fp32", scft_amd.PREC_FP32)):
+                ctx = scft_amd.Context(p, precision=prec)
+                ctx.calc_poly_density()
+                ctx.iterate(5)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                ctx.iterate(iters)
+                ctx.synchronize()
+                rate = iters / (time.perf_counter() - t0)
+                ctx.close()
+                b = b_iter(p.M, p.N, 16 if name == "fp64" else 8, 16)
+                out[f"{run} {p.Nx[0]}x{p.Nx[1]} N={p.N} Ng={p.Ng} {name}"] = {
+                    "iter_s": rate, "bytes_per_iter": b, "hbm_frac": b * rate / (HBM_PEAK_GBS * 1e9)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +158,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-published", action="store_true", help="skip the published 2D grids")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
@@ -219,6 +252,8 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_published:
+        out["published_2d"] = published_2d()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, fields, args.cpu_budget)
     print(planb, file=sys.stderr, flush=True)
