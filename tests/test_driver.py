@@ -354,3 +354,25 @@ def test_driver_reproduces_published_run(tmp_path):
         assert abs(g[1] - w[2]) <= 2.01e-6, (g, w)
         assert abs(g[2] - w[3]) <= 5e-5, (g, w)
     assert abs(got[-1][0] - H_pub) <= 2.01e-6
+
+
+def test_published_input_helper_round_trip(tmp_path):
+    # tests/golden/published_input.py writes a published bcp.input that the
+    # library's parser reads back as published (grid, plates, grafts)
+    sys.path.insert(0, GOLDEN)
+    try:
+        from published_input import published
+        from logfmt import log_template
+    finally:
+        sys.path.remove(GOLDEN)
+    text, H = published("dist-0.300")
+    assert H == pytest.approx(3999.628967, abs=1e-6)
+    (tmp_path / "bcp.input").write_text(text)
+    p = scft_amd.read_input(str(tmp_path / "bcp.input"), 2)
+    q = scft_amd.read_input(os.path.join(GOLDEN, "bcp_dist-0.300.input"), 2)
+    same_params(p, q)
+    # LOG masking: numbers become '#', per-rank and resume lines drop out,
+    # main-loop lines collapse
+    log = "Processor 3 reading\nRead wpl.res\nV = 4550.000000\nIter: 0, H=1.5, err=1e-3\nIter: 100, H=2, err=1e-4\n" \
+          "The program finished at Wed Jun 12 04:32:57 2019\n."
+    assert log_template(log) == ["V = #", "Iter: #, H=#, err=#", "The program finished at <date>", "."]
