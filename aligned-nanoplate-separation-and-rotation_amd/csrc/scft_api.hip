@@ -1233,13 +1233,20 @@ int graft_sites(scft_ctx* c, int si) {
   void *spec = nullptr, *g2 = nullptr;
   int rc = scratch_alloc(&spec);
   if (rc == SCFT_OK) rc = scratch_alloc(&g2);
+  // everything runs on the context stream, behind the work that still reads
+  // the FFT scratch; the host shape buffer is refilled only after the stream
+  // has drained its previous upload
   std::vector<cd> shape(M);
   for (int np = 0; np < std::min(s.p.n_exp_nr, 2) && rc == SCFT_OK; np++) {
+    if (hipStreamSynchronize(c->st) != hipSuccess) {
+      rc = fail(SCFT_EDEVICE, "stream failed");
+      break;
+    }
     for (int i = 0; i < M; i++) {
       const int nn[3] = {i % s.nx, (i / s.nx) % s.ny, i / (s.nx * s.ny)};
       shape[i] = plate_rho(s, np, nn, 1.05);
     }
-    if (hipMemcpy(c->scratch[si], shape.data(), m16, hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemcpyAsync(c->scratch[si], shape.data(), m16, hipMemcpyHostToDevice, c->st) != hipSuccess ||
         hipMemsetAsync(g2, 0, m16, c->st) != hipSuccess) {
       rc = fail(SCFT_EDEVICE, "copy failed");
       break;
@@ -1276,7 +1283,7 @@ int graft_sites(scft_ctx* c, int si) {
         after += e[i];
       }
       s.expl_total = after * s.dV;
-      if (hipMemcpy(s.dev.expl, e.data(), m16, hipMemcpyHostToDevice) != hipSuccess)
+      if (hipMemcpyAsync(s.dev.expl, e.data(), m16, hipMemcpyHostToDevice, c->st) != hipSuccess)
         rc = fail(SCFT_EDEVICE, "copy failed");
     }
   }
