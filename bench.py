@@ -77,12 +77,14 @@ def run_path(p, fields, prec, warmup, steps):
     return ctx, dt, H
 
 
-def cpu_baseline(p, fields, budget_s=20.0):
-    """FP64 oracle (reference dataflow) on the host cores, bounded sample."""
+def cpu_baseline(p, fields, budget_s=20.0, threads=None):
+    """FP64 oracle (reference dataflow) on the host cores (all of them, or
+    `threads`), bounded sample; the set-up density evaluation is not timed."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if threads is None:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     pyoracle.set_threads(threads)
     o = pyoracle.Oracle(p)
     o.set_fields(*fields)
@@ -256,6 +258,8 @@ def main():
         out["published_2d"] = published_2d()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, fields, args.cpu_budget)
+        # the same oracle on one core (at least one iteration)
+        out["cpu_baseline_1core"] = cpu_baseline(p, fields, args.cpu_budget / 2, threads=1)
     print(planb, file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
