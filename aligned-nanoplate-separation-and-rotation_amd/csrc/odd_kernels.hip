@@ -109,10 +109,11 @@ __device__ __forceinline__ void reg_dft(cpx<R>* a) {
 }
 
 // Column-group index of a block of a strided (y) pass.  Workgroups are
-// dispatched to the 8 XCDs in turn (block b on XCD b mod 8), and each XCD
-// has its own L2.  Neighbouring column groups share 128-byte lines, so the
-// blocks of one XCD get one contiguous run of groups: XCD x takes groups
-// [x q + min(x, r), ...) with nb = 8 q + r.  A bijection of [0, nb).
+// dispatched to the 8 XCDs in turn by linear id, and each XCD has its own
+// L2.  Neighbouring column groups share 128-byte lines, so the blocks of one
+// XCD get one contiguous run of groups: XCD x takes groups [x q + min(x, r),
+// ...) with nb = 8 q + r.  A bijection of [0, nb); the launcher pads nb to a
+// multiple of 8 so that block b of every job y sits on XCD b mod 8.
 __device__ __forceinline__ int xcd_group(int b, int nb) {
   const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
   return x * q + (x < r ? x : r) + k;

@@ -402,8 +402,12 @@ hipError_t launch_one(scft_ctx* c, const Launch& L, int step, hipStream_t st = n
   }
   if (L.fast == 3) {
     const int cpb = L.tiles[L.choice];
-    return scft::launch_odd_line(L.prec, L.mode, L.variants[L.choice], L.d, L.njobs,
-                                 (L.max_lines + cpb - 1) / cpb, step, st);
+    int gridx = (L.max_lines + cpb - 1) / cpb;
+    // strided passes group columns by XCD from the block index; workgroups
+    // go to XCDs by linear id x + gridx y, so gridx is padded to a multiple
+    // of 8 (the extra blocks exit) to keep that mapping for every job y
+    if (L.n1 == 1) gridx = (gridx + 7) & ~7;
+    return scft::launch_odd_line(L.prec, L.mode, L.variants[L.choice], L.d, L.njobs, gridx, step, st);
   }
   return scft::launch_pass(L.prec, L.mode, L.plane, L.d, L.njobs, L.gridx, L.tb, L.lds, step, st);
 }
