@@ -21,7 +21,13 @@
  *   density        sum rule nD*N = integral(rhoA+rhoB), uniform-field values
  *   1S update      fixed point <=> vanishing force; single-mode response
  *   H              uniform-field closed form
- *   loop / Euler   parity unpinned beyond the stage checks above: no
+ *   grafts, pairs  (tests/test_oracle_grafts.py) analytic graft-site
+ *                  gradient, graft sum rule and pairing, uniform-field
+ *                  Qga = -Ng c, force = dH/dw in every positive-chi field
+ *   1S loop        through the library, which equals this oracle stage by
+ *                  stage and reproduces the published runs' printed H at the
+ *                  reference's stop rule from their own inputs
+ *   Euler          parity unpinned beyond the stage checks above: no
  *                  reference trajectory exists (start fields missing).
  */
 #include <algorithm>
