@@ -1735,6 +1735,10 @@ int scft_grad_plates(scft_ctx* c, int sys, const double* axis, double* out) {
   if (rc) return rc;
   if (!axis || !out) return fail(SCFT_EINVAL, "null argument");
   SysHost& s = c->sys[sys];
+  // with grafted chains H also depends on the plates through the graft
+  // sites (-n ng_per_np Re integ(expl_grafts log q_g)); that term is not
+  // differentiated here, so the gradient would be incomplete
+  if (s.grafted) return fail(SCFT_EUNSUPPORTED, "plate derivatives are built for systems without grafted chains");
   double2 red[scft::RED_NSLOTS];
   if ((rc = copy_out(c, red, s.dev.red, sizeof red))) return rc;
   const cd Qs = cd(red[scft::RED_Q].x, red[scft::RED_Q].y) * (s.dV / s.info.V);

@@ -215,6 +215,10 @@ def test_pairs_reject_euler_and_noise():
         ctx.update_euler()
     with pytest.raises(scft_amd.ScftError):
         ctx.array("qg:99")
+    # plate derivatives do not cover the graft-site term: refused
+    ctx.calc_poly_density()
+    with pytest.raises(scft_amd.ScftError):
+        ctx.grad_plates()
     ctx.close()
     p.do_CL = 1
     with pytest.raises(scft_amd.ScftError):
