@@ -34,6 +34,13 @@ hipError_t launch_graft_init(int prec, const SysDev* s, int nsys, int maxM, hipS
 hipError_t launch_graft_density(int prec, const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_ik_mult(const SysDev* one, long M, const double2* in, double2* out, int d, hipStream_t st);
 hipError_t launch_sites_accum(long M, const double2* a, double2* acc, int mode, hipStream_t st);
+// graft-site derivatives: inflated shape / its plate derivatives, acc += a b,
+// block partial sums of (S / g)(L - Qga)
+hipError_t launch_site_shape(const SysDev* one, long M, int p, int lam, const double* axis, double scale,
+                             double2* out, hipStream_t st);
+hipError_t launch_cmul_acc(long M, const double2* a, const double2* b, double2* acc, hipStream_t st);
+hipError_t launch_site_grad_partial(int prec, const SysDev* one, int nblk, const double2* S, const double2* g,
+                                    double q_re, double q_im, double g2_min, double2* partials, hipStream_t st);
 hipError_t launch_cavity(const SysDev* s, int nsys, int maxM, hipStream_t st);
 hipError_t launch_plate_grad(const SysDev* one, int nblk, const double* axis, double lq_re, double lq_im,
                              double2* out, hipStream_t st);

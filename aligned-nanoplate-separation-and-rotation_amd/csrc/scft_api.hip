@@ -141,6 +141,7 @@ struct SysHost {
   int neg_bc = 0, neg_ac = 0;
   double chi_bc = 0.0, chi_ac = 0.0;
   cd expl_total = 0.0;  // integ(expl_grafts) after normalisation (unit)
+  cd expl_Z = 0.0;      // integ(sum_p |grad shape_p|), the normalisation
   std::vector<void*> allocs;
 };
 
@@ -1281,6 +1282,7 @@ int graft_sites(scft_ctx* c, int si) {
       cd total = 0.0;
       for (int i = 0; i < M; i++) total += e[i];
       total *= s.dV;
+      s.expl_Z = total;
       cd after = 0.0;
       for (int i = 0; i < M; i++) {
         e[i] /= total;
@@ -1293,6 +1295,112 @@ int graft_sites(scft_ctx* c, int si) {
   }
   (void)hipStreamSynchronize(c->st);
   for (void* p : tmp) (void)hipFree(p);
+  return rc;
+}
+
+// Plate derivatives of the graft term -n ng_per_np Re Qga, Qga =
+// integ(expl_grafts L) with L = log q_g[Ng-1] (code/calc_h.cpp:31-35), at
+// fixed fields.  expl_grafts = sum_p g_p / Z, g_p = |grad shape_p| of the
+// inflated plate shape (spectral gradient, as graft_sites computes it), Z =
+// integ(sum_p g_p), so for a parameter t of plate p
+//   d Qga / dt = (dV / Z) sum_r (d g_p / dt) (L - Qga),
+//   d g_p / dt = sum_d G_d G'_d / g_p,  G_d = D_d shape_p, G'_d = D_d d shape_p / dt,
+// the exact derivative of the discrete graft sites.  out[4 p + k]: Re part of
+// dH/dc_k (k < 3) and dH/dtheta (k = 3) about `axis`.
+int graft_site_grad(scft_ctx* c, int si, const double* axis, cd Qga, double* out) {
+  SysHost& s = c->sys[si];
+  const int M = s.M, dim = s.p.dim;
+  const size_t m16 = (size_t)M * 16;
+  const SysDev* row = c->d_sys + si;
+  std::vector<void*> tmp;
+  auto get = [&](double2** p) -> int {
+    if (hipMalloc((void**)p, m16) != hipSuccess) return fail(SCFT_ENOMEM, "no memory for the graft-site derivative");
+    tmp.push_back(*p);
+    return SCFT_OK;
+  };
+  double2 *spec = nullptr, *G[3] = {nullptr, nullptr, nullptr}, *g2 = nullptr, *gq = nullptr, *S = nullptr,
+          *part = nullptr;
+  int rc = get(&spec);
+  for (int d = 0; d < dim && rc == SCFT_OK; d++) rc = get(&G[d]);
+  if (rc == SCFT_OK) rc = get(&g2);
+  if (rc == SCFT_OK) rc = get(&gq);
+  if (rc == SCFT_OK) rc = get(&S);
+  const int nblk = s.dev.nblk;
+  if (rc == SCFT_OK && hipMalloc((void**)&part, (size_t)nblk * sizeof(double2)) != hipSuccess)
+    rc = fail(SCFT_ENOMEM, "no memory for the graft-site derivative");
+  if (part) tmp.push_back(part);
+  double2* scr = (double2*)c->scratch[si];
+  // D_d of the field in scr, into dst (the spectrum is kept in spec)
+  auto grad_of_scr = [&](bool fwd, int d, double2* dst) -> int {
+    int r;
+    if (fwd) {
+      if ((r = run_program(c, c->prog_fft_fwd[si], 0))) return r;
+      HIPCK(hipMemcpyAsync(spec, scr, m16, hipMemcpyDeviceToDevice, c->st));
+    }
+    HIPCK(scft::launch_ik_mult(row, M, spec, scr, d, c->st));
+    if ((r = run_program(c, c->prog_fft_inv[si], 0))) return r;
+    HIPCK(hipMemcpyAsync(dst, scr, m16, hipMemcpyDeviceToDevice, c->st));
+    return SCFT_OK;
+  };
+  const double pref = -double(s.p.n_exp_nr) * s.info.ng_per_np;
+  for (int k = 0; k < 8; k++) out[k] = 0.0;
+  std::vector<double2> h(nblk);
+  for (int p = 0; p < s.dev.n_plates && rc == SCFT_OK; p++) {
+    // G_d and g_p of the inflated shape
+    if (scft::launch_site_shape(row, M, p, -1, axis, 1.05, scr, c->st) != hipSuccess ||
+        hipMemsetAsync(g2, 0, m16, c->st) != hipSuccess || hipMemsetAsync(gq, 0, m16, c->st) != hipSuccess) {
+      rc = fail(SCFT_EDEVICE, "graft-site derivative kernel failed");
+      break;
+    }
+    for (int d = 0; d < dim && rc == SCFT_OK; d++) {
+      if ((rc = grad_of_scr(d == 0, d, G[d]))) break;
+      if (scft::launch_sites_accum(M, G[d], g2, 0, c->st) != hipSuccess) rc = fail(SCFT_EDEVICE, "kernel failed");
+    }
+    if (rc == SCFT_OK && scft::launch_sites_accum(M, g2, gq, 1, c->st) != hipSuccess)
+      rc = fail(SCFT_EDEVICE, "kernel failed");
+    // round-off zeros of g: |g| <= 1e-12 max |g|
+    double g2_min = 0.0;
+    if (rc == SCFT_OK) {
+      std::vector<double2> gh(M);
+      if (hipMemcpyAsync(gh.data(), gq, m16, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+          hipStreamSynchronize(c->st) != hipSuccess) {
+        rc = fail(SCFT_EDEVICE, "copy failed");
+        break;
+      }
+      double gmax2 = 0.0;
+      for (int i = 0; i < M; i++) gmax2 = std::max(gmax2, gh[i].x * gh[i].x + gh[i].y * gh[i].y);
+      g2_min = 1e-24 * gmax2;
+    }
+    for (int lam = 0; lam < 4 && rc == SCFT_OK; lam++) {
+      if (lam >= dim && lam < 3) continue;  // no centre coordinate along an absent axis
+      if (scft::launch_site_shape(row, M, p, lam, axis, 1.05, scr, c->st) != hipSuccess ||
+          hipMemsetAsync(S, 0, m16, c->st) != hipSuccess) {
+        rc = fail(SCFT_EDEVICE, "graft-site derivative kernel failed");
+        break;
+      }
+      for (int d = 0; d < dim && rc == SCFT_OK; d++) {
+        if ((rc = grad_of_scr(d == 0, d, g2))) break;  // g2 reused as G'_d
+        if (scft::launch_cmul_acc(M, G[d], g2, S, c->st) != hipSuccess) rc = fail(SCFT_EDEVICE, "kernel failed");
+      }
+      if (rc) break;
+      if (scft::launch_site_grad_partial(c->prec, row, nblk, S, gq, Qga.real(), Qga.imag(), g2_min, part, c->st) !=
+          hipSuccess) {
+        rc = fail(SCFT_EDEVICE, "kernel failed");
+        break;
+      }
+      if (hipMemcpyAsync(h.data(), part, (size_t)nblk * sizeof(double2), hipMemcpyDeviceToHost, c->st) !=
+              hipSuccess ||
+          hipStreamSynchronize(c->st) != hipSuccess) {
+        rc = fail(SCFT_EDEVICE, "copy failed");
+        break;
+      }
+      cd sum = 0.0;
+      for (int b = 0; b < nblk; b++) sum += cd(h[b].x, h[b].y);
+      out[4 * p + lam] = pref * (sum * s.dV / s.expl_Z).real();
+    }
+  }
+  (void)hipStreamSynchronize(c->st);
+  for (void* q : tmp) (void)hipFree(q);
   return rc;
 }
 
@@ -1735,10 +1843,6 @@ int scft_grad_plates(scft_ctx* c, int sys, const double* axis, double* out) {
   if (rc) return rc;
   if (!axis || !out) return fail(SCFT_EINVAL, "null argument");
   SysHost& s = c->sys[sys];
-  // with grafted chains H also depends on the plates through the graft
-  // sites (-n ng_per_np Re integ(expl_grafts log q_g)); that term is not
-  // differentiated here, so the gradient would be incomplete
-  if (s.grafted) return fail(SCFT_EUNSUPPORTED, "plate derivatives are built for systems without grafted chains");
   double2 red[scft::RED_NSLOTS];
   if ((rc = copy_out(c, red, s.dev.red, sizeof red))) return rc;
   const cd Qs = cd(red[scft::RED_Q].x, red[scft::RED_Q].y) * (s.dV / s.info.V);
@@ -1749,6 +1853,11 @@ int scft_grad_plates(scft_ctx* c, int sys, const double* axis, double* out) {
   for (int k = 0; k < 8; k++) {
     out[2 * k] = k < 4 * s.dev.n_plates ? g[k].x : 0.0;
     out[2 * k + 1] = k < 4 * s.dev.n_plates ? g[k].y : 0.0;
+  }
+  if (s.grafted) {
+    double gg[8];
+    if ((rc = graft_site_grad(c, sys, axis, qga(s, red), gg))) return rc;
+    for (int k = 0; k < 4 * s.dev.n_plates; k++) out[2 * k] += gg[k];
   }
   return SCFT_OK;
 }

@@ -779,9 +779,10 @@ __device__ __forceinline__ void cross3(const double* a, const double* b, double*
 // d erfc(z) / dz
 __device__ __forceinline__ double derfc(double z) { return -1.1283791670955125739 * exp(-z * z); }
 
+// `scale` inflates the length and radius (1.05 for the graft sites)
 template <bool GRAD>
 __device__ __forceinline__ double plate_eval(const SysDev& s, int p, long i, const double* axis, double* dc,
-                                             double* dth) {
+                                             double* dth, double scale = 1.0) {
   const int nn[3] = {(int)(i % s.nx), (int)((i / s.nx) % s.ny), (int)(i / ((long)s.nx * s.ny))};
   const PlateDev& P = s.plate[p];
   double dr[3] = {0.0, 0.0, 0.0};
@@ -790,16 +791,16 @@ __device__ __forceinline__ double plate_eval(const SysDev& s, int p, long i, con
     if (dr[d] >= 0.5 * s.Lb[d]) dr[d] -= s.Lb[d];
     else if (dr[d] < -0.5 * s.Lb[d]) dr[d] += s.Lb[d];
   }
-  const double xi = s.xi_nr;
+  const double xi = s.xi_nr, len = scale * s.len_nr, rad = scale * s.R_nr;
   const double ud = dot3(P.u, dr);
-  const double a = (fabs(ud) - 0.5 * s.len_nr) / xi;
+  const double a = (fabs(ud) - 0.5 * len) / xi;
   const double ea = erfc(a);
   const double sud = ud > 0.0 ? 1.0 : (ud < 0.0 ? -1.0 : 0.0);
   double du[3] = {0.0, 0.0, 0.0};
   if (GRAD) cross3(axis, P.u, du);
   if (s.np_type == SCFT_NP_SQUARE && s.dim == 3) {
     const double a1 = dot3(P.e1, dr), a2 = dot3(P.e2, dr);
-    const double b1 = (fabs(a1) - s.R_nr) / xi, b2 = (fabs(a2) - s.R_nr) / xi;
+    const double b1 = (fabs(a1) - rad) / xi, b2 = (fabs(a2) - rad) / xi;
     const double e1v = erfc(b1), e2v = erfc(b2);
     if (GRAD) {
       const double s1 = a1 > 0.0 ? 1.0 : (a1 < 0.0 ? -1.0 : 0.0);
@@ -834,7 +835,7 @@ __device__ __forceinline__ double plate_eval(const SysDev& s, int p, long i, con
       dsc_dth = -ud * dot3(du, dr) / sc;
     }
   }
-  const double b = (sc - s.R_nr) / xi;
+  const double b = (sc - rad) / xi;
   const double eb = erfc(b);
   if (GRAD) {
     const double ga = 0.25 * derfc(a) * eb / xi;
@@ -900,6 +901,65 @@ __global__ __launch_bounds__(256) void k_plate_grad2(const SysDev* __restrict__ 
     if (threadIdx.x == 0) out[k] = dscale(sh[0], s.dV);
     __syncthreads();
   }
+}
+
+// Graft-site derivatives (the plate dependence of -n ng_per_np Re Qga):
+// the inflated shape of plate p (lam < 0) or its derivative with respect to
+// centre coordinate lam (0..2) or to a rotation about `axis` (lam 3)
+__global__ __launch_bounds__(256) void k_site_shape(const SysDev* __restrict__ sysp, int p, int lam, double ax,
+                                                    double ay, double az, double scale, double2* __restrict__ out) {
+  const SysDev& s = *sysp;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= s.M) return;
+  const double axis[3] = {ax, ay, az};
+  double dc[3], dth, v;
+  if (lam < 0) {
+    v = plate_eval<false>(s, p, i, nullptr, nullptr, nullptr, scale);
+  } else {
+    plate_eval<true>(s, p, i, axis, dc, &dth, scale);
+    v = lam < 3 ? dc[lam] : dth;
+  }
+  out[i] = make_double2(v, 0.0);
+}
+
+// acc += a b
+__global__ __launch_bounds__(256) void k_cmul_acc(long M, const double2* __restrict__ a,
+                                                  const double2* __restrict__ b, double2* __restrict__ acc) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < M) acc[i] = dadd(acc[i], dmul(a[i], b[i]));
+}
+
+// block partial sums of (S / g) (log q_g[Ng-1] - Ng c - Qga) over the points
+// where Re expl_grafts > 0 (the points Qga sums over); S = sum_d G_d G'_d, so
+// S / g is the derivative of g = |grad shape|.  Where g vanishes (|g|^2 <=
+// g2_min: the plate centre and symmetry lines, at round-off level) g has a
+// kink whose one-sided derivatives are +-|G'|; those points add 0, the
+// symmetric derivative
+template <typename R>
+__global__ __launch_bounds__(256) void k_site_grad_partial(const SysDev* __restrict__ sysp,
+                                                           const double2* __restrict__ S,
+                                                           const double2* __restrict__ g, double q_re, double q_im,
+                                                           double g2_min, double2* __restrict__ partials) {
+  const SysDev& s = *sysp;
+  __shared__ double2 sh[256];
+  double2 acc = make_double2(0.0, 0.0);
+  const double c = s.red[RED_SHIFT].y;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < s.M; i += (long)gridDim.x * 256) {
+    const double2 gv = g[i];
+    const double gg = gv.x * gv.x + gv.y * gv.y;
+    if (s.expl[i].x <= 0.0 || gg <= g2_min) continue;
+    const cpx<R> v = ld<R>(s.qg, (long)(s.Ng - 1) * s.M + i);
+    const double vx = v.x, vy = v.y;
+    const double2 L = make_double2(0.5 * log(vx * vx + vy * vy) - (double)s.Ng * c - q_re, atan2(vy, vx) - q_im);
+    acc = dadd(acc, dmul(ddiv(S[i], gv), L));
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = dadd(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
 }
 
 __global__ __launch_bounds__(256) void k_scale_real(double* a, long n, double f) {
@@ -1017,6 +1077,29 @@ hipError_t launch_plate_grad(const SysDev* one, int nblk, const double* axis, do
                              double2* out, hipStream_t st) {
   hipLaunchKernelGGL(k_plate_grad1, dim3(nblk), dim3(256), 0, st, one, axis[0], axis[1], axis[2], lq_re, lq_im);
   hipLaunchKernelGGL(k_plate_grad2, dim3(1), dim3(256), 0, st, one, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_site_shape(const SysDev* one, long M, int p, int lam, const double* axis, double scale,
+                             double2* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_site_shape, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st, one, p, lam,
+                     axis ? axis[0] : 0.0, axis ? axis[1] : 0.0, axis ? axis[2] : 0.0, scale, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_cmul_acc(long M, const double2* a, const double2* b, double2* acc, hipStream_t st) {
+  hipLaunchKernelGGL(k_cmul_acc, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st, M, a, b, acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_site_grad_partial(int prec, const SysDev* one, int nblk, const double2* S, const double2* g,
+                                    double q_re, double q_im, double g2_min, double2* partials, hipStream_t st) {
+  if (prec == 0)
+    hipLaunchKernelGGL(k_site_grad_partial<double>, dim3(nblk), dim3(256), 0, st, one, S, g, q_re, q_im, g2_min,
+                       partials);
+  else
+    hipLaunchKernelGGL(k_site_grad_partial<float>, dim3(nblk), dim3(256), 0, st, one, S, g, q_re, q_im, g2_min,
+                       partials);
   return hipGetLastError();
 }
 

@@ -177,9 +177,10 @@ int scft_set_plates(scft_ctx* ctx, int sys, const double* centres,
  * At a saddle point (dH/dw = 0) these are the total derivatives: the force
  * and torque on the plates.  out[8 p + 2 k + {0,1}] = Re, Im of dH/dc_k
  * (k = 0..2) and of dH/dtheta (k = 3), p = 0..1.  Needs a density
- * evaluation (Qd) of the current fields.  SCFT_EUNSUPPORTED for systems with
- * grafted chains, whose H also depends on the plates through the graft
- * sites. */
+ * evaluation (Qd) of the current fields.  With grafted chains H also depends
+ * on the plates through the graft sites (-n ng_per_np Re Qga,
+ * code/calc_h.cpp:31-35); that term is included (real parts), as the exact
+ * derivative of the discrete graft sites of code/initialize.cpp:160-196. */
 int scft_grad_plates(scft_ctx* ctx, int sys, const double* axis, double* out);
 
 /* Complex-Langevin noise of the last update, in k-space (the reference's

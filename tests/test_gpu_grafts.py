@@ -215,10 +215,6 @@ def test_pairs_reject_euler_and_noise():
         ctx.update_euler()
     with pytest.raises(scft_amd.ScftError):
         ctx.array("qg:99")
-    # plate derivatives do not cover the graft-site term: refused
-    ctx.calc_poly_density()
-    with pytest.raises(scft_amd.ScftError):
-        ctx.grad_plates()
     ctx.close()
     p.do_CL = 1
     with pytest.raises(scft_amd.ScftError):
@@ -280,3 +276,68 @@ def test_published_log_bookkeeping():
                 ctx.close()
     finally:
         del os.environ["SCFT_NO_TUNE"]
+
+
+def test_grafted_plate_gradient_vs_fd_fixed_fields():
+    # at fixed fields the plates enter H through rho_np, nD and, with grafted
+    # chains, the graft sites (-n ng_per_np Re Qga).  scft_grad_plates against
+    # central differences (Richardson from h and h/2) of the library's H with
+    # the plates moved (graft sites recomputed), and of the oracle's H
+    p = small_grafted()
+    f = physical_fields(p, 11)
+    ctx = scft_amd.Context(p)
+    for n, a in zip(FIELDS, f):
+        ctx.set_array(n, a)
+    ctx.calc_poly_density()
+    g = ctx.grad_plates(axis=(0.0, 0.0, 1.0))
+    L = [p.L[0], p.L[1]]
+    c0 = np.array([[p.exp_nr_c[j][d] for d in range(3)] for j in range(2)])
+    u0 = np.array([[p.exp_nr_u[j][d] for d in range(3)] for j in range(2)])
+
+    def H_lib(c, u):
+        ctx.set_plates(c.reshape(-1), u.reshape(-1))
+        ctx.calc_poly_density()
+        return ctx.calc_H()[0].real
+
+    def rot(u, t):
+        cs, sn = np.cos(t), np.sin(t)
+        return np.array([cs * u[0] - sn * u[1], sn * u[0] + cs * u[1], 0.0])
+
+    def moved(j, k, h):
+        c, u = c0.copy(), u0.copy()
+        if k < 2:
+            c[j, k] += h / L[k]
+        else:
+            u[j] = rot(u0[j], h)
+        return c, u
+
+    def fd(j, k, h, H):
+        return (H(*moved(j, k, h)) - H(*moved(j, k, -h))) / (2 * h)
+
+    # b units for the centres, radians for the rotation (1e-4 rad moves the
+    # plate ends, ~12 b from the centre, by about the translation step)
+    hs = (1e-3, 1e-4)
+    worst = 0.0
+    for j in range(2):
+        for k in range(3):
+            h = hs[0] if k < 2 else hs[1]
+            f1, f2 = fd(j, k, h, H_lib), fd(j, k, h / 2, H_lib)
+            rich = (4 * f2 - f1) / 3
+            ana = g[j, k if k < 2 else 3].real
+            say(f" plate {j} {'xyt'[k]}: analytic {ana:.10e} fd {f1:.10e} richardson {rich:.10e}")
+            worst = max(worst, abs(ana - rich) / max(abs(rich), 1e-3))
+            assert abs(ana - rich) <= 1e-6 * max(abs(rich), 1e-2), (j, k, ana, rich)
+    ctx.close()
+    # the oracle's H at the same fields, plate 0 moved along y
+    def H_oracle(c, u):
+        q = pyoracle.Oracle(_moved(p, c.reshape(-1), u.reshape(-1)))
+        for n, a in zip(FIELDS, f):
+            q.set_array(n, a)
+        q.calc_poly_density()
+        h = q.calc_H().real
+        q.close()
+        return h
+    fo1, fo2 = fd(0, 1, 1e-3, H_oracle), fd(0, 1, 5e-4, H_oracle)
+    ro = (4 * fo2 - fo1) / 3
+    say(f" oracle plate 0 y: richardson {ro:.10e} analytic {g[0, 1].real:.10e}")
+    assert abs(g[0, 1].real - ro) <= 1e-6 * max(abs(ro), 1e-2)
