@@ -37,6 +37,8 @@ def main():
     ap.add_argument("--out", required=True)
     ap.add_argument("--max-seconds", type=float, default=1000.0)
     ap.add_argument("--fp32", action="store_true")
+    ap.add_argument("--forces", action="store_true",
+                    help="also record dH/dc (plate centres, b units) and dH/dtheta (about z) at each stop")
     args = ap.parse_args()
     last = json.load(open(os.path.join(HERE, "log_last_rows.json")))
     ps, Hpub = [], []
@@ -75,6 +77,12 @@ def main():
                 stop[i] = dict(run=args.runs[i], iter=it, H=H[i].real, mlogQd=-math.log(abs(Q)),
                                mlogQg=-ps[i].n_exp_nr * infos[i].ng_per_np * Qg.real, err=float(err[i]),
                                H_published=Hpub[i], published_last_row=last[args.runs[i]][-1])
+                if args.forces:
+                    g = ctx.grad_plates(axis=(0.0, 0.0, 1.0), sys=i)
+                    stop[i]["dH_dc"] = [[g[j, k].real for k in range(3)] for j in range(ps[i].n_exp_nr)]
+                    stop[i]["dH_dtheta"] = [g[j, 3].real for j in range(ps[i].n_exp_nr)]
+                    stop[i]["centres"] = [[ps[i].exp_nr_c[j][k] * ps[i].L[k] for k in range(2)]
+                                          for j in range(ps[i].n_exp_nr)]
                 print(f" {args.runs[i]} stopped at {it}: H {H[i].real:.6f} published {Hpub[i]}", flush=True)
         if all(s is not None for s in stop) or time.time() - t0 > args.max_seconds:
             break
