@@ -119,8 +119,13 @@ int scft_read_input(const char* path, int dim, scft_params* out);
 /* Create a context holding `n_systems` systems on HIP device `device`.
  * `params` points to n_systems structs (boxes may differ).  Performs the
  * set-up of initialize_1/initialize_2 (code/initialize.cpp:45-412): smearing
- * and bond kernels, plate densities, volume bookkeeping, Debye functions and
- * the initial fields of init_fields (code/initialize.cpp:419-565). */
+ * and bond kernels, plate densities, graft sites, volume bookkeeping, Debye
+ * functions and the initial fields of init_fields (code/initialize.cpp:
+ * 419-565), then times the size-specialised pass variants on the context's
+ * own buffers and keeps the fastest.  Environment switches read here:
+ * SCFT_NO_TUNE (keep the first variant of each pass, no timing),
+ * SCFT_TUNE_PICK=k (take variant k, or the last, of every pass; for tests),
+ * SCFT_ONE_STREAM (run the q and q-dagger chains on one stream). */
 int scft_create(const scft_params* params, int n_systems, int device,
                 int precision, scft_ctx** out);
 int scft_destroy(scft_ctx* ctx);
