@@ -342,22 +342,38 @@ def write_input(p: ScftParams, path: str) -> None:
         f.write("\n".join(L) + "\n")
 
 
+def _write_res_grid(path: str, a, dim: int, Nx, L) -> None:
+    a = np.asarray(a, dtype=np.complex128)
+    nx = Nx[0]
+    ny = Nx[1] if dim > 1 else 1
+    M = int(np.prod(Nx[:dim]))
+    if a.size != M:
+        raise ValueError(f"{path}: {a.size} values for a grid of {M} points")
+    dx = [L[k] / Nx[k] for k in range(dim)]
+    out = []
+    for i in range(M):
+        n = (i % nx, (i // nx) % ny, i // (nx * ny))
+        c = " ".join(repr(float(n[k] * dx[k])) for k in range(dim))
+        out.append(f"{c} {float(a[i].real)!r} {float(a[i].imag)!r}")
+        if dim == 2 and n[0] == nx - 1:
+            out.append("")
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
 def write_res(path: str, a, p: ScftParams) -> None:
     """Writes a start field as a .res file for read_one_resume_file
     (code/io_utils.cpp:137-177): one point per line, the `dim` coordinates
     n dx then Re Im, x fastest, a blank line after each x row in 2D."""
-    a = np.asarray(a, dtype=np.complex128)
-    nx, ny, nz = p.grid()
-    dx = [p.L[k] / p.Nx[k] for k in range(p.dim)]
-    out = []
-    for i in range(p.M):
-        n = (i % nx, (i // nx) % ny, i // (nx * ny))
-        c = " ".join(repr(float(n[k] * dx[k])) for k in range(p.dim))
-        out.append(f"{c} {float(a[i].real)!r} {float(a[i].imag)!r}")
-        if p.dim == 2 and n[0] == nx - 1:
-            out.append("")
-    with open(path, "w") as f:
-        f.write("\n".join(out) + "\n")
+    _write_res_grid(path, a, p.dim, [p.Nx[k] for k in range(3)], [p.L[k] for k in range(3)])
+
+
+def bin_to_res(bin_path: str, res_path: str) -> None:
+    """Turns a field dump (<name>.p0.bin of write_data_bin) into the .res
+    start file read_resume_files reads (code/io_utils.cpp:183-240), so a run
+    resumes from the dumps of an earlier one."""
+    dim, nx, L, _, data = read_bin(bin_path)
+    _write_res_grid(res_path, data, dim, nx, L)
 
 
 def read_bin(path: str):

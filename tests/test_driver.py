@@ -376,3 +376,25 @@ def test_published_input_helper_round_trip(tmp_path):
     log = "Processor 3 reading\nRead wpl.res\nV = 4550.000000\nIter: 0, H=1.5, err=1e-3\nIter: 100, H=2, err=1e-4\n" \
           "The program finished at Wed Jun 12 04:32:57 2019\n."
     assert log_template(log) == ["V = #", "Iter: #, H=#, err=#", "The program finished at <date>", "."]
+
+
+def test_bin_dump_converts_to_res(tmp_path):
+    # a .p0.bin dump in the reference's layout (int dim, int Nx[dim],
+    # double L[dim], int nprocs, int ML, ML complex) becomes a .res file whose
+    # lines carry the coordinates and the values of the dump
+    import struct
+    nx, ny, L = (6, 4), 2, (3.0, 2.0)
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(24) + 1j * rng.standard_normal(24)
+    raw = struct.pack("i", 2) + struct.pack("2i", *nx) + struct.pack("2d", *L) + struct.pack("2i", 1, 24)
+    raw += a.astype(np.complex128).tobytes()
+    (tmp_path / "wpl.p0.bin").write_bytes(raw)
+    scft_amd.bin_to_res(str(tmp_path / "wpl.p0.bin"), str(tmp_path / "wpl.res"))
+    lines = [l.split() for l in open(tmp_path / "wpl.res") if l.strip()]
+    assert len(lines) == 24
+    vals = np.array([[float(x) for x in l] for l in lines])
+    assert np.array_equal(vals[:, 2] + 1j * vals[:, 3], a)
+    assert vals[7, 0] == 0.5 and vals[7, 1] == 0.5       # point (1, 1): x = 1 * 3/6, y = 1 * 2/4
+    # a blank line after each x row, as the reference's text writer does
+    # (code/io_utils.cpp:340-341)
+    assert open(tmp_path / "wpl.res").read().count("\n\n") == 4
