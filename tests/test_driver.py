@@ -398,3 +398,35 @@ def test_bin_dump_converts_to_res(tmp_path):
     # a blank line after each x row, as the reference's text writer does
     # (code/io_utils.cpp:340-341)
     assert open(tmp_path / "wpl.res").read().count("\n\n") == 4
+
+
+@pytest.mark.gpu
+def test_driver_resumes_from_its_own_dumps(tmp_path):
+    # run A: 20 iterations of the grafted 2D box, field dumps at iteration 20;
+    # the dumps become .res start files (bin_to_res) and run B starts from
+    # them: its starting H is run A's H at iteration 20 (all seven fields)
+    need_driver()
+    p = scft_amd.read_input(os.path.join(GOLDEN, "bcp_dist-0.300.input"), 2)
+    p.Nda, p.Ndb, p.Ng = 10, 10, 4
+    p.Nx[0], p.Nx[1] = 48, 80
+    p.L[0], p.L[1] = 24.0, 40.0
+    p.itermax, p.print_freq, p.error_tol = 20, 10, 1e-14
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    scft_amd.write_input(p, str(a / "bcp.input"))
+    print("scft_run resume A", flush=True)
+    ra = subprocess.run([DRIVER, "--dim", "2", "--seed", "7"], cwd=a, capture_output=True, text=True, timeout=300)
+    assert ra.returncode == 0, ra.stdout[-2000:] + ra.stderr[-2000:]
+    H20 = scft_amd.read_data_dat(str(a / "data.dat"))[-1]
+    assert H20[0] == 20
+    names = ["wpl", "wabp", "wabm", "wbcp", "wbcm", "wacp", "wacm"]
+    for n in names:
+        scft_amd.bin_to_res(str(a / f"{n}.p0.bin"), str(b / f"{n}.res"))
+    scft_amd.write_input(p, str(b / "bcp.input"))
+    print("scft_run resume B", flush=True)
+    rb = subprocess.run([DRIVER, "--dim", "2", "--seed", "8"], cwd=b, capture_output=True, text=True, timeout=300)
+    assert rb.returncode == 0, rb.stdout[-2000:] + rb.stderr[-2000:]
+    assert rb.stdout.count("Processor 0 reading from line 1.") == 7
+    start = float(rb.stdout.split("Starting H: ")[1].split()[0])
+    assert abs(start - H20[1]) <= 2e-6, (start, H20[1])
