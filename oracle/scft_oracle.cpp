@@ -2,9 +2,10 @@
  * ORACLE -- test infrastructure only.
  *
  * A clean-room FP64 CPU restatement of the reference SCFT hot path
- * (rar-ensemble/aligned-nanoplate-separation-and-rotation, code/).  It is the
- * checker for the HIP library: tests, the smoke check and the CPU-baseline leg
- * of the benchmark load it; the product never links or calls it.
+ * (rar-ensemble/aligned-nanoplate-separation-and-rotation, code/).  The HIP
+ * library's results are compared against it: tests, the smoke check and the
+ * CPU-baseline leg of the benchmark load it; the product never links or calls
+ * it.
  *
  * It follows the reference dataflow on purpose (it is also the timed CPU
  * baseline): forward FFT scaled by 1/M, a complex exp(-w) evaluated at every

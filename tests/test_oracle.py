@@ -1,4 +1,4 @@
-"""Pins the FP64 CPU oracle before it is trusted as the checker.
+"""Pins the FP64 CPU oracle before the library is compared against it.
 
 Each check is chosen so that a plausible slip (a dropped term, a wrong sign or
 index, a transposed operand) fails it:
