@@ -119,6 +119,9 @@ __device__ __forceinline__ int xcd_group(int b, int nb) {
   return x * q + (x < r ? x : r) + k;
 }
 
+// LDS bound of a two-step block (gfx950: 160 KiB per workgroup)
+constexpr size_t kOddLdsMax = 160 * 1024;
+
 template <int N1, int N2, int AXIS, int CPB, int LW>
 struct OddCfg {
   static constexpr int n = N1 * N2;
@@ -140,7 +143,9 @@ __global__ __launch_bounds__((OddCfg<N1, N2, AXIS, CPB, LW>::T)) void k_odd_line
                                                                                 int step) {
   using C = OddCfg<N1, N2, AXIS, CPB, LW>;
   constexpr int n = C::n;
-  __shared__ cpx<R> lds[CPB * n];
+  // CPB lines of n values, dynamic: up to 160 KiB per workgroup on gfx950
+  extern __shared__ __align__(16) unsigned char odd_smem[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(odd_smem);
   const PassDesc& d = descs[blockIdx.y];
   if (d.steps_active && step_contour(step) >= d.steps_active) return;
   const long M = (long)d.nx * d.ny * d.nz;
@@ -259,7 +264,14 @@ __global__ __launch_bounds__((OddCfg<N1, N2, AXIS, CPB, LW>::T)) void k_odd_line
 template <typename R, int MODE, int N1, int N2, int AXIS, int CPB, int LW>
 hipError_t launch_odd_m(const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
   using C = OddCfg<N1, N2, AXIS, CPB, LW>;
-  hipLaunchKernelGGL((k_odd_line<R, MODE, N1, N2, AXIS, CPB, LW>), dim3(gridx, njobs), dim3(C::T), 0, st, d, step);
+  constexpr size_t lds = sizeof(cpx<R>) * CPB * C::n;
+  static bool attr = false;
+  if (!attr && lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)k_odd_line<R, MODE, N1, N2, AXIS, CPB, LW>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  attr = true;
+  hipLaunchKernelGGL((k_odd_line<R, MODE, N1, N2, AXIS, CPB, LW>), dim3(gridx, njobs), dim3(C::T), lds, st, d,
+                     step);
   return hipGetLastError();
 }
 
@@ -442,13 +454,13 @@ hipError_t launch_stock_v(int prec, int mode, const PassDesc* d, int njobs, int 
   return hipErrorInvalidValue;
 }
 
-// the precisions whose static LDS (CPB lines of n values) fits 64 KiB
+// the precisions whose LDS (CPB lines of n values) fits a workgroup
 template <int NN, int N1, int N2, int AX, int CP, int LW>
 hipError_t launch_odd_v(int prec, int mode, const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
-  if constexpr ((size_t)CP * NN * 16 <= 64 * 1024) {
+  if constexpr ((size_t)CP * NN * 16 <= kOddLdsMax) {
     if (prec == 0) return launch_odd_t<double, N1, N2, AX, CP, LW>(mode, d, njobs, gridx, step, st);
   }
-  if constexpr ((size_t)CP * NN * 8 <= 64 * 1024) {
+  if constexpr ((size_t)CP * NN * 8 <= kOddLdsMax) {
     if (prec == 1) return launch_odd_t<float, N1, N2, AX, CP, LW>(mode, d, njobs, gridx, step, st);
   }
   return hipErrorInvalidValue;
@@ -476,9 +488,10 @@ hipError_t launch_odd_v(int prec, int mode, const PassDesc* d, int njobs, int gr
 int odd_line_variants(int prec, int n, int axis, int* ids, int* cpb, int max_ids) {
   int k = 0, id = 0;
   const size_t esz = prec == 0 ? 16 : 8;
-  // static LDS of one block: CPB lines of n complex values, at most 64 KiB
-#define X(nn, a, b, ax, cp, lw)                                                   \
-  if (n == nn && axis == ax && k < max_ids && (size_t)cp * nn * esz <= 64 * 1024) { \
+  // LDS of one block: CPB lines of n complex values (two-step: dynamic, up
+  // to 160 KiB; Stockham: static, up to 64 KiB)
+#define X(nn, a, b, ax, cp, lw)                                                    \
+  if (n == nn && axis == ax && k < max_ids && (size_t)cp * nn * esz <= kOddLdsMax) { \
     cpb[k] = cp;                                                                  \
     ids[k++] = id;                                                                \
   }                                                                               \
