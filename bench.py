@@ -36,8 +36,6 @@ import time
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "aligned-nanoplate-separation-and-rotation_amd"))
 
-import numpy as np  # noqa: E402
-
 import scft_amd  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0

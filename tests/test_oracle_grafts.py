@@ -21,7 +21,6 @@ import numpy as np
 import pytest
 
 import pyoracle
-import scft_amd
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 FIELDS = ["wpl", "wabp", "wabm", "wbcp", "wbcm", "wacp", "wacm"]
