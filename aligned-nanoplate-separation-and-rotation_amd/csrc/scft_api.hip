@@ -1669,6 +1669,7 @@ int scft_get_smeared(scft_ctx* c, int sys, double* smwa, double* smwb) {
 int scft_get_cavity(scft_ctx* c, int sys, double* rho) {
   int rc = check_ctx(c, sys);
   if (rc) return rc;
+  if (!rho) return fail(SCFT_EINVAL, "null output");
   SysHost& s = c->sys[sys];
   return copy_out(c, rho, s.dev.rho_np, (size_t)s.M * 16);
 }
@@ -1676,6 +1677,7 @@ int scft_get_cavity(scft_ctx* c, int sys, double* rho) {
 int scft_get_Q(scft_ctx* c, int sys, double* Q) {
   int rc = check_ctx(c, sys);
   if (rc) return rc;
+  if (!Q) return fail(SCFT_EINVAL, "null output");
   SysHost& s = c->sys[sys];
   double2 red[scft::RED_NSLOTS];
   if ((rc = copy_out(c, red, s.dev.red, sizeof red))) return rc;
@@ -1772,6 +1774,7 @@ int scft_set_array(scft_ctx* c, int sys, const char* name, const double* in) {
 int scft_get_Qga(scft_ctx* c, int sys, double* Qga) {
   int rc = check_ctx(c, sys);
   if (rc) return rc;
+  if (!Qga) return fail(SCFT_EINVAL, "null output");
   SysHost& s = c->sys[sys];
   if (!s.grafted) return fail(SCFT_EINVAL, "system has no grafted chains");
   double2 red[scft::RED_NSLOTS];
@@ -1785,6 +1788,7 @@ int scft_get_Qga(scft_ctx* c, int sys, double* Qga) {
 int scft_set_plates(scft_ctx* c, int sys, const double* centres, const double* u) {
   int rc = check_ctx(c, sys);
   if (rc) return rc;
+  if (!centres || !u) return fail(SCFT_EINVAL, "null argument");
   SysHost& s = c->sys[sys];
   const int np = s.dev.n_plates;
   if (np == 0) return fail(SCFT_EINVAL, "system has no plates");
@@ -1865,6 +1869,7 @@ int scft_grad_plates(scft_ctx* c, int sys, const double* axis, double* out) {
 int scft_fft(scft_ctx* c, int sys, double* inout, int dir) {
   int rc = check_ctx(c, sys);
   if (rc) return rc;
+  if (!inout) return fail(SCFT_EINVAL, "null argument");
   if (dir != -1 && dir != 1) return fail(SCFT_EINVAL, "dir must be -1 or +1");
   SysHost& s = c->sys[sys];
   const size_t b = (size_t)s.M * 16;
@@ -2080,7 +2085,10 @@ int scft_iterate(scft_ctx* c, int n_iter) {
     HIPCK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     rc = stage_iteration(c, scheme);
     hipError_t e = hipStreamEndCapture(c->st, &g);
-    if (rc) return rc;
+    if (rc) {
+      if (e == hipSuccess) (void)hipGraphDestroy(g);
+      return rc;
+    }
     if (e != hipSuccess) return fail(SCFT_EDEVICE, "graph capture failed");
     e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
