@@ -161,8 +161,8 @@ def main():
     ap.add_argument("--no-published", action="store_true", help="skip the published 2D grids")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
-              "(independent systems shard across GPUs without exchange; no multi-GPU run here).")
+        print("bench.py measures one GPU: the hot path runs on a single MI355X "
+              "(independent systems would shard across GPUs without exchange; no multi-GPU path).")
         sys.exit(2)
 
     # headline: FP32 propagator, a batch of 16 production systems in one context
