@@ -28,8 +28,11 @@
  *   1S loop        through the library, which equals this oracle stage by
  *                  stage and reproduces the published runs' printed H at the
  *                  reference's stop rule from their own inputs
- *   Euler          parity unpinned beyond the stage checks above: no
- *                  reference trajectory exists (start fields missing).
+ *   Euler          (tests/test_oracle_euler.py) the noise-free step is
+ *                  w - lambda dH/dw in every field for chi_AB > 0, with the
+ *                  reference's lam_imag / lam_real assignment; the CL
+ *                  noise is pinned statistically only, and no reference
+ *                  trajectory exists (start fields missing).
  */
 #include <algorithm>
 #include <cmath>
