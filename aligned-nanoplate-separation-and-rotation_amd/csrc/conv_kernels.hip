@@ -320,19 +320,13 @@ template <typename R, int MODE, int NX, int NY, int GX, int GY, int D>
 static hipError_t launch_plane_m(const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
   using C = PlaneCfg<NX, NY, GX, GY>;
   constexpr size_t lds = plane_lds_bytes<R, NX, NY, GX, GY>();
-  static bool attr = false;
+  static bool attr[64] = {};
   if constexpr (D) {
-    if (!attr && lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)k_plane_direct<R, MODE, NX, NY, GX, GY>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr = true;
+    raise_lds_limit((const void*)k_plane_direct<R, MODE, NX, NY, GX, GY>, lds, attr);
     hipLaunchKernelGGL((k_plane_direct<R, MODE, NX, NY, GX, GY>), dim3(gridx, njobs), dim3(C::T), lds, st, d,
                        step);
   } else {
-    if (!attr && lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)k_plane_op<R, MODE, NX, NY, GX, GY>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr = true;
+    raise_lds_limit((const void*)k_plane_op<R, MODE, NX, NY, GX, GY>, lds, attr);
     hipLaunchKernelGGL((k_plane_op<R, MODE, NX, NY, GX, GY>), dim3(gridx, njobs), dim3(C::T), lds, st, d,
                        step);
   }
@@ -355,14 +349,11 @@ template <typename R, int MODE, int NZ, int G, int TL, int D>
 static hipError_t launch_z_m(const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
   using C = ZCfg<NZ, G, TL>;
   constexpr size_t lds = z_variant_lds<R, NZ, G, TL, D>();
-  static bool attr = false;
+  static bool attr[64] = {};
   if constexpr (D) {
     hipLaunchKernelGGL((k_z_direct<R, MODE, NZ, G, TL>), dim3(gridx, njobs), dim3(C::T), lds, st, d, step);
   } else {
-    if (!attr && lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)k_z_op<R, MODE, NZ, G, TL>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr = true;
+    raise_lds_limit((const void*)k_z_op<R, MODE, NZ, G, TL>, lds, attr);
     hipLaunchKernelGGL((k_z_op<R, MODE, NZ, G, TL>), dim3(gridx, njobs), dim3(C::T), lds, st, d, step);
   }
   return hipGetLastError();

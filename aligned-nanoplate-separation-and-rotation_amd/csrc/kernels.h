@@ -8,6 +8,20 @@ namespace scft {
 struct PassDesc;
 struct SysDev;
 
+// Dynamic LDS above 64 KiB needs hipFuncAttributeMaxDynamicSharedMemorySize
+// on the kernel, and the attribute belongs to the current device's copy of
+// it.  `done` holds one flag per device for one kernel instantiation
+// (devices 0..63; others set the attribute on every launch).
+inline void raise_lds_limit(const void* kernel, size_t bytes, bool* done) {
+  if (bytes <= 64 * 1024) return;
+  int dev = -1;
+  (void)hipGetDevice(&dev);
+  const bool cached = dev >= 0 && dev < 64;
+  if (cached && done[dev]) return;
+  (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (cached) done[dev] = true;
+}
+
 // prec: 0 = complex128, 1 = complex64.  mode: MODE_CONV / MODE_FWD / MODE_INV.
 // plane: k_plane (x and y of one xy plane per block) or k_line (one axis).
 hipError_t launch_pass(int prec, int mode, bool plane, const PassDesc* d, int njobs, int gridx,

@@ -265,11 +265,8 @@ template <typename R, int MODE, int N1, int N2, int AXIS, int CPB, int LW>
 hipError_t launch_odd_m(const PassDesc* d, int njobs, int gridx, int step, hipStream_t st) {
   using C = OddCfg<N1, N2, AXIS, CPB, LW>;
   constexpr size_t lds = sizeof(cpx<R>) * CPB * C::n;
-  static bool attr = false;
-  if (!attr && lds > 64 * 1024)
-    (void)hipFuncSetAttribute((const void*)k_odd_line<R, MODE, N1, N2, AXIS, CPB, LW>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  attr = true;
+  static bool attr[64] = {};
+  raise_lds_limit((const void*)k_odd_line<R, MODE, N1, N2, AXIS, CPB, LW>, lds, attr);
   hipLaunchKernelGGL((k_odd_line<R, MODE, N1, N2, AXIS, CPB, LW>), dim3(gridx, njobs), dim3(C::T), lds, st, d,
                      step);
   return hipGetLastError();

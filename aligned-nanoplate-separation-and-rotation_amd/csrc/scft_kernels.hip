@@ -972,15 +972,11 @@ __global__ __launch_bounds__(256) void k_scale_real(double* a, long n, double f)
 // ---------------------------------------------------------------------------
 template <typename R, int MODE, int TB, bool PLANE>
 static void set_lds_attr() {
-  static bool done = false;
-  if (done) return;
-  done = true;
+  static bool done[64] = {};
   if (PLANE)
-    (void)hipFuncSetAttribute((const void*)k_plane<R, MODE, TB, 32>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    raise_lds_limit((const void*)k_plane<R, MODE, TB, 32>, 160 * 1024, done);
   else
-    (void)hipFuncSetAttribute((const void*)k_line<R, MODE, TB, 32>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    raise_lds_limit((const void*)k_line<R, MODE, TB, 32>, 160 * 1024, done);
 }
 
 template <typename R, int MODE>
