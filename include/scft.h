@@ -11,7 +11,7 @@
  * SCFT_EINVAL, except for the arrays of the set/get calls that move several
  * arrays at once (scft_set_fields, scft_get_fields, scft_get_density,
  * scft_get_smeared, scft_get_noise, scft_get_averages), where a NULL array is
- * skipped.  scft_last_error() describes the last failure of the calling
+ * skipped, and scft_destroy(NULL), a no-op.  scft_last_error() describes the last failure of the calling
  * thread.
  *
  * Threading: one host thread per context.  Calls on one context are ordered
