@@ -8,11 +8,11 @@
  * length M in the reference layout, linear index x + Nx*(y + Ny*z), x fastest
  * (code/array_utils.cpp:80-87).  Every call returns SCFT_OK (0) or a negative
  * error code; nothing calls exit().  A NULL context or array pointer returns
- * SCFT_EINVAL, except for the arrays of the set/get calls that move several
+ * SCFT_EINVAL, with two exceptions: the set/get calls that move several
  * arrays at once (scft_set_fields, scft_get_fields, scft_get_density,
- * scft_get_smeared, scft_get_noise, scft_get_averages), where a NULL array is
- * skipped, and scft_destroy(NULL), a no-op.  scft_last_error() describes the last failure of the calling
- * thread.
+ * scft_get_smeared, scft_get_noise, scft_get_averages) skip a NULL array, and
+ * scft_destroy(NULL) is a no-op.  scft_last_error() describes the last
+ * failure of the calling thread.
  *
  * Threading: one host thread per context.  Calls on one context are ordered
  * on the context's HIP stream; the scft_get_* calls synchronise.
