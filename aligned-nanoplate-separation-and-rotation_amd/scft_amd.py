@@ -512,8 +512,14 @@ class Context:
         self._check(self.lib.scft_get_info(self.h, sys, ctypes.byref(s)), "scft_get_info")
         return s
 
+    def _sized(self, a, n, what):
+        if a.size != n:
+            raise ValueError(f"{what}: expected {n} values, got {a.size}")
+        return a
+
     def set_fields(self, wpl, wabp, wabm, sys=0):
-        a = [np.ascontiguousarray(x, dtype=np.complex128) for x in (wpl, wabp, wabm)]
+        a = [self._sized(np.ascontiguousarray(x, dtype=np.complex128), self.M(sys), "set_fields")
+             for x in (wpl, wabp, wabm)]
         self._check(self.lib.scft_set_fields(self.h, sys, *[_dptr(x) for x in a]),
                     "scft_set_fields")
 
@@ -547,8 +553,9 @@ class Context:
 
     def set_plates(self, centres, us, sys=0):
         """centres, us: (n_plates, 3) fractional centres and orientations."""
-        c = np.ascontiguousarray(np.asarray(centres, dtype=np.float64).reshape(-1))
-        u = np.ascontiguousarray(np.asarray(us, dtype=np.float64).reshape(-1))
+        n = 3 * self.params[sys].n_exp_nr
+        c = self._sized(np.ascontiguousarray(np.asarray(centres, dtype=np.float64).reshape(-1)), n, "centres")
+        u = self._sized(np.ascontiguousarray(np.asarray(us, dtype=np.float64).reshape(-1)), n, "orientations")
         self._check(self.lib.scft_set_plates(self.h, sys, _dptr(c), _dptr(u)), "scft_set_plates")
 
     def set_geometry(self, d: float, theta_deg: float, sys=0):
@@ -560,7 +567,7 @@ class Context:
 
     def grad_plates(self, axis=(0.0, 1.0, 0.0), sys=0):
         """(n, 4) complex: dH/dc_x, dH/dc_y, dH/dc_z (b units), dH/dtheta per plate."""
-        a = np.ascontiguousarray(np.asarray(axis, dtype=np.float64))
+        a = self._sized(np.ascontiguousarray(np.asarray(axis, dtype=np.float64).reshape(-1)), 3, "axis")
         out = np.empty(16)
         self._check(self.lib.scft_grad_plates(self.h, sys, _dptr(a), _dptr(out)), "scft_grad_plates")
         g = out[0::2] + 1j * out[1::2]
@@ -604,7 +611,7 @@ class Context:
         return complex(out[0], out[1])
 
     def fft(self, a, direction: int, sys=0):
-        x = np.array(a, dtype=np.complex128, copy=True)
+        x = self._sized(np.array(a, dtype=np.complex128, copy=True, order="C"), self.M(sys), "fft")
         self._check(self.lib.scft_fft(self.h, sys, _dptr(x), direction), "scft_fft")
         return x
 
