@@ -29,10 +29,11 @@
  *                  stage and reproduces the published runs' printed H at the
  *                  reference's stop rule from their own inputs
  *   Euler          (tests/test_oracle_euler.py) the noise-free step is
- *                  w - lambda dH/dw in every field for chi_AB > 0, with the
- *                  reference's lam_imag / lam_real assignment; the CL
- *                  noise is pinned statistically only, and no reference
- *                  trajectory exists (start fields missing).
+ *                  w - lambda dH/dw in every field of positive chi (w+, AB,
+ *                  and with grafted chains BC and AC), with the reference's
+ *                  lam_imag / lam_real assignment; the CL noise is pinned
+ *                  statistically only, and no reference trajectory exists
+ *                  (start fields missing).
  */
 #include <algorithm>
 #include <cmath>
@@ -805,19 +806,46 @@ struct Oracle {
     calc_poly_density();
   }
 
-  /* update_Euler, code/Euler_update.cpp:5-213 (AB pair). */
+  /* One graft pair of update_Euler: BC (rho = rhodb,
+   * code/Euler_update.cpp:108-143) or AC (rho = rhoda, :146-181), explicit
+   * steps, the + field with lam_imag and the - field with lam_real
+   * (:122,133,160,171); densities in k-space. */
+  void euler_pair(std::vector<cd>& wp, std::vector<cd>& wm, const std::vector<cd>& rho, double chi, bool neg) {
+    const cd I(0.0, 1.0);
+    const double rho0 = p.rho0, li = p.lam_imag, lr = p.lam_real;
+    const cd sp = neg ? cd(-1.0) : I;   // rho and rhog_exp in the + force
+    const cd sm = neg ? I : cd(-1.0);   // rho in the - force
+    const cd sg = neg ? -I : cd(1.0);   // rhog_exp in the - force
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < M; i++) {
+      cd F = 2.0 * rho0 * wp[i] / chi + hhat[i] * (rho[i] * sp + rhog_exp[i] * sp);
+      cd numer = wp[i] - li * F;
+      if (p.do_CL) numer += etap[i];
+      wp[i] = numer;
+      F = 2.0 * rho0 * wm[i] / chi + hhat[i] * (rho[i] * sm + rhog_exp[i] * sg);
+      numer = wm[i] - lr * F;
+      if (p.do_CL) numer += etam[i];
+      wm[i] = numer;
+    }
+  }
+
+  /* update_Euler, code/Euler_update.cpp:5-213: w+, the AB pair and the BC /
+   * AC pairs of the grafted chains. */
   void update_euler() {
     const cd I(0.0, 1.0);
     const double rho0 = p.rho0, kappa = p.kappa, li = p.lam_imag, lr = p.lam_real;
     if (nD > 0.0) { fft_fwd(rhoda); fft_fwd(rhodb); }
+    if (p.sigma > 0.0 && p.n_exp_nr > 0) fft_fwd(rhog_exp);
     fft_fwd(wpl);
     if (chi_ab != 0.0) { fft_fwd(wabm); fft_fwd(wabp); }
+    if (chi_bc != 0.0) { fft_fwd(wbcm); fft_fwd(wbcp); }
+    if (chi_ac != 0.0) { fft_fwd(wacm); fft_fwd(wacp); }
     if (p.do_CL) noise(etap, li);
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < M; i++) {
       cd evar = (i == 0) ? 1.0 : 0.0;
       cd F = (kappa <= 0.0 ? cd(0.0) : rho0 * wpl[i] / kappa) +
-             I * rho0 * (surfH[i] + exp_nrH[i] - evar) + I * hhat[i] * (rhoda[i] + rhodb[i]);
+             I * rho0 * (surfH[i] + exp_nrH[i] - evar) + I * hhat[i] * (rhoda[i] + rhodb[i] + rhog_exp[i]);
       cd numer = wpl[i] - li * F;
       if (p.do_CL) numer += etap[i];
       wpl[i] = numer;
@@ -841,8 +869,22 @@ struct Oracle {
     } else {
       for (long i = 0; i < M; i++) wabp[i] = wabm[i] = 0.0;
     }
+    if (chi_bc != 0.0) {
+      if (p.do_CL) { noise(etap, li); noise(etam, lr); }
+      euler_pair(wbcp, wbcm, rhodb, chi_bc, neg_bc);
+    } else {
+      for (long i = 0; i < M; i++) wbcp[i] = wbcm[i] = 0.0;
+    }
+    if (chi_ac != 0.0) {
+      if (p.do_CL) { noise(etap, lr); noise(etam, li); }
+      euler_pair(wacp, wacm, rhoda, chi_ac, neg_ac);
+    } else {
+      for (long i = 0; i < M; i++) wacp[i] = wacm[i] = 0.0;
+    }
     fft_bck(wpl);
     if (chi_ab != 0.0) { fft_bck(wabm); fft_bck(wabp); }
+    if (chi_bc != 0.0) { fft_bck(wbcm); fft_bck(wbcp); }
+    if (chi_ac != 0.0) { fft_bck(wacm); fft_bck(wacp); }
     calc_poly_density();
   }
 
